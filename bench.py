@@ -1,0 +1,203 @@
+"""Benchmark: hybrid key-switch throughput at N = 2^16, L = 24, dnum = 3, w = 60
+on one MI355X, with NTT limb-GB/s and the HBM roofline of the NTT pass kernel.
+
+One step = one replay of a hipGraph holding the six HEAL calls over a batch of
+B independent key-switches (own x and own evaluation key each).  The plain
+per-call form (B = 1, eager launches) and the batched eager form are timed
+beside it, and all three must agree bit for bit.  Inputs are resident in HBM
+before timing.  The cpu_baseline leg times the reference's CPU implementation
+(compiled from its sources into oracle/_ref) on one key-switch of the same
+shape and checks its output against the GPU's.
+
+  python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--limbs", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-samples", type=int, default=2)
+    return ap.parse_args()
+
+
+def timed(fn, reps, torch):
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1000.0
+
+
+def cpu_baseline_leg(ks_gpu, samples):
+    """Times the reference key-switch (oracle/_ref) on instance 0 of the batch."""
+    import ctypes
+
+    import torch
+
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_lib import hview, reference
+
+    ref = reference()
+    if ref is None:
+        return None, None
+    N, L, D = ks_gpu.n, ks_gpu.L, ks_gpu.dnum
+    prm = ks_gpu.prm
+    x = ks_gpu.x[0].cpu()
+    evk = ks_gpu.evk[:, 0].cpu().contiguous()
+    host = {k: getattr(prm, k).cpu() for k in ("p", "perm", "twiddles", "inv_twiddles", "m_inv")}
+    perms = ks_gpu.perms.cpu()
+    out = torch.zeros((2, N, L), dtype=torch.int64)
+    times = []
+    for _ in range(samples):
+        sec = ctypes.c_double(0)
+        st = ref.ref_keyswitch(hview(x), hview(evk), hview(host["p"]), hview(host["perm"]), hview(host["twiddles"]),
+                               hview(host["inv_twiddles"]), hview(host["m_inv"]), hview(perms), D, ks_gpu.w,
+                               hview(out), ctypes.byref(sec))
+        if st != 0:
+            raise RuntimeError("reference key-switch failed: " + ref.ref_last_error().decode())
+        times.append(sec.value)
+    times.sort()
+    med = times[len(times) // 2]
+    # bit-exactness of the GPU output against the reference (valid where the
+    # reference is defined: its INTT scaling overflows int64 for q > ~2^31.5,
+    # so the comparison uses the oracle's exact chain instead when it differs)
+    gpu_out = ks_gpu.out[0].cpu()
+    return med, bool(torch.equal(out, gpu_out))
+
+
+def oracle_chain_equal(ks_gpu):
+    """The oracle's exact chain on instance 0 (the checker; not timed)."""
+    import torch
+
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from keyswitch_oracle import oracle_keyswitch
+
+    ref_out = oracle_keyswitch(ks_gpu, 0)
+    return bool(torch.equal(ref_out, ks_gpu.out[0].cpu()))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by limb with no "
+              "exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    torch.cuda.init()
+    N, L, B = args.n, args.limbs, args.batch
+    batched = KeySwitchBatch(n=N, limbs=L, batch=B, seed=0)
+    plain = KeySwitchBatch(n=N, limbs=L, batch=1, seed=0, primes=batched.primes)
+    plain.x.copy_(batched.x[:1])
+    plain.evk.copy_(batched.evk[:, :1])
+
+    # correctness gate: plain == batched eager == graph replay, bit for bit
+    plain.run()
+    batched.run()
+    torch.cuda.synchronize()
+    ref0 = batched.out[0].clone()
+    assert torch.equal(plain.out[0], ref0), "plain and batched forms differ"
+    batched.capture()
+    batched.out.zero_()
+    batched.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(batched.out[0], ref0), "graph replay differs from eager"
+
+    for _ in range(args.warmup):
+        batched.replay()
+        plain.run()
+    torch.cuda.synchronize()
+
+    t_graph = timed(batched.replay, args.steps, torch)
+    t_batched = timed(batched.run, args.steps, torch)
+    t_plain = timed(plain.run, args.steps * 2, torch) / 2.0
+    ks_graph = args.steps * B / t_graph
+    ks_batched = args.steps * B / t_batched
+    ks_plain = args.steps / t_plain
+
+    # NTT pass kernel: forward digit NTTs of the batch (B*dnum*L limbs of N points);
+    # each of the 2 launches reads and writes every limb once.
+    limbs_fwd = B * batched.dnum * L
+    bytes_per_launch = limbs_fwd * 2 * N * 8
+    reps = 10
+    dig_view = batched.digits.permute(0, 1, 3, 2)
+    t_ntt = timed(lambda: batched.fwd(dig_view, batched.dntt), reps, torch)
+    n_launch = 2 if N > 4096 else 1
+    avg_launch = t_ntt / (reps * n_launch)
+    achieved = bytes_per_launch / avg_launch / 1e9
+    ntt_limb_gbs = limbs_fwd * 2 * N * 8 * reps / t_ntt / 1e9
+    limbs_inv = B * 2 * L
+    t_intt = timed(lambda: batched.inv(batched.acc.view(B * 2, N, 1, L), batched.coef), reps, torch)
+    intt_limb_gbs = limbs_inv * 2 * N * 8 * reps / t_intt / 1e9
+
+    result = {
+        "metric": "key-switches/sec (N=2^16, L=24, dnum=3, w=60), batched hipGraph form; NTT limb-GB/s beside",
+        "value": round(ks_graph, 2),
+        "unit": "key-switches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_graph / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded uniform residues mod 24 primes of 50-60 bits; seeded evaluation keys)",
+        "config": {"workload": f"hybrid key-switch N={N} L={L} dnum=3 w=60, Galois g=5 permute", "batch_per_step": B,
+                   "form": "graph-batched", "plain_ks_per_s": round(ks_plain, 2),
+                   "batched_eager_ks_per_s": round(ks_batched, 2),
+                   "ntt_limb_GBps": round(ntt_limb_gbs, 1), "intt_limb_GBps": round(intt_limb_gbs, 1),
+                   "keyswitch_algorithmic_bytes": batched.algorithmic_bytes() // B,
+                   "keyswitch_chain_bytes": batched.chain_bytes() // B},
+        "roofline": {"bound": "hbm", "kernel": "k_ntt_pass (forward, 2 launches per NTT)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4)},
+        "cpu_baseline": None,
+    }
+    try:
+        result["config"]["oracle_bit_exact"] = oracle_chain_equal(plain)
+    except Exception as e:  # the checker is optional on the box; never the measured path
+        result["config"]["oracle_bit_exact"] = f"unchecked: {e}"
+    if not args.no_cpu_baseline:
+        try:
+            med, equal = cpu_baseline_leg(plain, args.cpu_samples)
+            if med is not None:
+                cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+                result["cpu_baseline"] = {
+                    "value": round(1.0 / med, 4), "unit": "key-switches/s", "cores": cores, "kind": "reference",
+                    "sample": f"{args.cpu_samples} full-size key-switches through the reference example_impl "
+                              f"(median of compute time, {cores} OpenMP threads)",
+                    "gpu_over_cpu": round(ks_graph * med, 1),
+                    "reference_output_equal": equal,
+                }
+        except Exception as e:
+            result["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(result), flush=True)
+    batched.close()
+    plain.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,294 @@
+"""ctypes binding of the HEAL MI355X C ABI (include/heal_amd.h).
+
+Tensors are torch tensors on a ROCm device; each call passes a view descriptor
+(device pointer, dtype, dims, strides) and the current torch stream.  Status
+codes become the Python exceptions that pybind11 raises for the reference's
+C++ exceptions: std::invalid_argument -> ValueError, std::out_of_range ->
+IndexError, std::runtime_error -> RuntimeError, std::bad_alloc -> MemoryError.
+
+The library must have been built (``python -m heal_amd.build``); importing this
+module without it raises ImportError — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+HX_MAX_DIMS = 8
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("HEAL_AMD_LIB", os.path.join(HERE, "libheal_amd.so"))
+
+HX_INT32, HX_INT64, HX_FLOAT64 = 0, 1, 2
+HX_OK, HX_ERR_INVALID_ARGUMENT, HX_ERR_OUT_OF_RANGE, HX_ERR_RUNTIME, HX_ERR_BAD_ALLOC = range(5)
+HX_PERMS_TRUSTED = 1
+
+_DTYPE = {torch.int32: HX_INT32, torch.int64: HX_INT64, torch.float64: HX_FLOAT64}
+
+
+class HxView(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("dtype", ctypes.c_int32),
+        ("ndim", ctypes.c_int32),
+        ("dims", ctypes.c_int64 * HX_MAX_DIMS),
+        ("strides", ctypes.c_int64 * HX_MAX_DIMS),
+    ]
+
+
+class HxDeviceStatus(ctypes.Structure):
+    _fields_ = [("gdecomp_overflow", ctypes.c_uint64), ("index_out_of_range", ctypes.c_uint64)]
+
+
+VP = ctypes.POINTER(HxView)
+I64 = ctypes.c_int64
+U64 = ctypes.c_uint64
+STREAM = ctypes.c_void_p
+
+# name -> argtypes (all return int status)
+_PROTOS = {
+    "hx_status_poll": [STREAM, ctypes.POINTER(HxDeviceStatus)],
+    "hx_malloc": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, STREAM],
+    "hx_free": [ctypes.c_void_p, STREAM],
+    "hx_memset_zero": [ctypes.c_void_p, ctypes.c_size_t, STREAM],
+    "hx_memcpy_h2d": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, STREAM],
+    "hx_memcpy_d2h": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, STREAM],
+    "hx_memcpy_d2d": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, STREAM],
+    "hx_stream_sync": [STREAM],
+    "hx_strided_copy": [VP, VP, STREAM],
+    "hx_modmul_ttt": [VP, VP, VP, VP, STREAM],
+    "hx_modmul_ttc": [VP, VP, I64, VP, STREAM],
+    "hx_modmul_tct": [VP, I64, VP, VP, STREAM],
+    "hx_modmul_tcc": [VP, I64, I64, VP, STREAM],
+    "hx_modsum_ttt": [VP, VP, VP, VP, STREAM],
+    "hx_modsum_ttc": [VP, VP, I64, VP, STREAM],
+    "hx_modsum_tct": [VP, I64, VP, VP, STREAM],
+    "hx_modsum_tcc": [VP, I64, I64, VP, STREAM],
+    "hx_mod_tt": [VP, VP, VP, STREAM],
+    "hx_mod_tc": [VP, I64, VP, STREAM],
+    "hx_mod_ct": [I64, VP, VP, STREAM],
+    "hx_axis_modsum": [VP, VP, VP, I64, STREAM],
+    "hx_g_decomposition": [VP, VP, U64, U64, STREAM],
+    "hx_ntt": [VP, VP, VP, VP, VP, VP, VP, STREAM],
+    "hx_intt": [VP, VP, VP, VP, VP, VP, VP, VP, STREAM],
+    "hx_permute": [VP, VP, VP, I64, I64, STREAM],
+    "hx_permute_ex": [VP, VP, VP, I64, I64, ctypes.c_uint32, STREAM],
+    "hx_ntt_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, VP, VP, VP, VP, STREAM],
+    "hx_ntt_plan_exec": [ctypes.c_void_p, VP, VP, STREAM],
+    "hx_ntt_plan_destroy": [ctypes.c_void_p],
+}
+
+# optional entry points (vendor extensions added over time)
+_OPTIONAL = {
+    "hx_keyswitch_fused": None,
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"HEAL MI355X library not found at {LIB_PATH}; build it with `python -m heal_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.hx_last_error.restype = ctypes.c_char_p
+    lib.hx_last_error.argtypes = []
+    lib.hx_abi_version.restype = ctypes.c_int
+    return lib
+
+
+lib = _load()
+
+
+def exported_symbols() -> list[str]:
+    return sorted(list(_PROTOS) + ["hx_last_error", "hx_abi_version"])
+
+
+class HealError(RuntimeError):
+    pass
+
+
+def check(code: int) -> None:
+    if code == HX_OK:
+        return
+    msg = lib.hx_last_error().decode(errors="replace")
+    if code == HX_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if code == HX_ERR_OUT_OF_RANGE:
+        raise IndexError(msg)
+    if code == HX_ERR_BAD_ALLOC:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def view(t: torch.Tensor | None):
+    """View descriptor of a tensor (None -> NULL pointer)."""
+    if t is None:
+        return None
+    if t.dim() > HX_MAX_DIMS:
+        raise ValueError("tensors of more than 8 dims are not supported")
+    if t.dtype not in _DTYPE:
+        raise TypeError(f"unsupported dtype {t.dtype}")
+    v = HxView()
+    v.data = t.data_ptr() if t.numel() else 0
+    v.dtype = _DTYPE[t.dtype]
+    v.ndim = t.dim()
+    for i, (d, s) in enumerate(zip(t.shape, t.stride())):
+        v.dims[i] = d
+        v.strides[i] = s
+    return ctypes.byref(v)
+
+
+def stream_of(t: torch.Tensor | None = None) -> int:
+    dev = t.device if t is not None and t.is_cuda else None
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def _need_cuda(*ts):
+    for t in ts:
+        if isinstance(t, torch.Tensor) and not t.is_cuda:
+            raise ValueError("HEAL MI355X ops take device tensors")
+
+
+# ------------------------------------------------------------ op wrappers
+def modmul_ttt(a, b, p, out):
+    _need_cuda(a, b, p, out)
+    check(lib.hx_modmul_ttt(view(a), view(b), view(p), view(out), stream_of(out)))
+    return out
+
+
+def modmul_ttc(a, b, p: int, out):
+    _need_cuda(a, b, out)
+    check(lib.hx_modmul_ttc(view(a), view(b), int(p), view(out), stream_of(out)))
+    return out
+
+
+def modmul_tct(a, b: int, p, out):
+    _need_cuda(a, p, out)
+    check(lib.hx_modmul_tct(view(a), int(b), view(p), view(out), stream_of(out)))
+    return out
+
+
+def modmul_tcc(a, b: int, p: int, out):
+    _need_cuda(a, out)
+    check(lib.hx_modmul_tcc(view(a), int(b), int(p), view(out), stream_of(out)))
+    return out
+
+
+def modsum_ttt(a, b, p, out):
+    _need_cuda(a, b, p, out)
+    check(lib.hx_modsum_ttt(view(a), view(b), view(p), view(out), stream_of(out)))
+    return out
+
+
+def modsum_ttc(a, b, p: int, out):
+    _need_cuda(a, b, out)
+    check(lib.hx_modsum_ttc(view(a), view(b), int(p), view(out), stream_of(out)))
+    return out
+
+
+def modsum_tct(a, b: int, p, out):
+    _need_cuda(a, p, out)
+    check(lib.hx_modsum_tct(view(a), int(b), view(p), view(out), stream_of(out)))
+    return out
+
+
+def modsum_tcc(a, b: int, p: int, out):
+    _need_cuda(a, out)
+    check(lib.hx_modsum_tcc(view(a), int(b), int(p), view(out), stream_of(out)))
+    return out
+
+
+def mod_tt(a, b, out):
+    _need_cuda(a, b, out)
+    check(lib.hx_mod_tt(view(a), view(b), view(out), stream_of(out)))
+    return out
+
+
+def mod_tc(a, b: int, out):
+    _need_cuda(a, out)
+    check(lib.hx_mod_tc(view(a), int(b), view(out), stream_of(out)))
+    return out
+
+
+def mod_ct(a: int, b, out):
+    _need_cuda(b, out)
+    check(lib.hx_mod_ct(int(a), view(b), view(out), stream_of(out)))
+    return out
+
+
+def axis_modsum(a, p, out, axis: int):
+    _need_cuda(a, p, out)
+    check(lib.hx_axis_modsum(view(a), view(p), view(out), int(axis), stream_of(out)))
+    return out
+
+
+def g_decomposition(a, out, power: int, base_bits: int):
+    _need_cuda(a, out)
+    check(lib.hx_g_decomposition(view(a), view(out), int(power), int(base_bits), stream_of(out)))
+    return out
+
+
+def ntt(a, p, perm, twiddles, out, log2p=None, mu=None):
+    _need_cuda(a, p, perm, twiddles, out)
+    check(lib.hx_ntt(view(a), view(p), view(perm), view(twiddles), view(log2p), view(mu), view(out), stream_of(out)))
+    return out
+
+
+def intt(a, p, perm, inv_twiddles, m_inv, out, log2p=None, mu=None):
+    _need_cuda(a, p, perm, inv_twiddles, m_inv, out)
+    check(lib.hx_intt(view(a), view(p), view(perm), view(inv_twiddles), view(m_inv), view(log2p), view(mu),
+                      view(out), stream_of(out)))
+    return out
+
+
+def permute(a, perms, out, elementwise_axis: int, perm_axis: int, trusted: bool = False):
+    _need_cuda(a, perms, out)
+    if trusted:
+        check(lib.hx_permute_ex(view(a), view(perms), view(out), int(elementwise_axis), int(perm_axis),
+                                HX_PERMS_TRUSTED, stream_of(out)))
+    else:
+        check(lib.hx_permute(view(a), view(perms), view(out), int(elementwise_axis), int(perm_axis), stream_of(out)))
+    return out
+
+
+def strided_copy(src, dst):
+    _need_cuda(src, dst)
+    check(lib.hx_strided_copy(view(src), view(dst), stream_of(dst)))
+    return dst
+
+
+def status_poll(stream: int | None = None) -> dict:
+    st = HxDeviceStatus()
+    check(lib.hx_status_poll(stream if stream is not None else stream_of(), ctypes.byref(st)))
+    return {"gdecomp_overflow": st.gdecomp_overflow, "index_out_of_range": st.index_out_of_range}
+
+
+class NttPlan:
+    """Device snapshot of (p, perm, twiddles[, m_inv]) for repeated transforms."""
+
+    def __init__(self, p, perm, twiddles, m_inv=None, inverse: bool = False):
+        _need_cuda(p, perm, twiddles, m_inv)
+        self._h = ctypes.c_void_p()
+        self.inverse = bool(inverse)
+        check(lib.hx_ntt_plan_create(ctypes.byref(self._h), int(self.inverse), view(p), view(perm), view(twiddles),
+                                     view(m_inv), stream_of(p)))
+
+    def __call__(self, a, out):
+        _need_cuda(a, out)
+        check(lib.hx_ntt_plan_exec(self._h, view(a), view(out), stream_of(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            check(lib.hx_ntt_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,217 @@
+// hx_arith.hpp — exact 64-bit modular arithmetic for gfx950 (and the host).
+//
+// The reference computes every modular product with a double-width integer
+// and `%` (example_impl/typing.h:8-20: int32 -> int64, int64 -> __int128).
+// CDNA4 has no 128-bit divide, so this file reduces with precomputed
+// reciprocals instead:
+//
+//  * General moduli (modops, on-load reduction): division by an invariant
+//    64-bit divisor after Möller & Granlund, "Improved division by invariant
+//    integers", IEEE Trans. Computers 60(2), 2011 (algorithm "div_2by1"): the
+//    divisor is normalised (top bit set) and its reciprocal
+//    v = floor((2^128-1)/d) - 2^64 is computed once per modulus.
+//  * Fixed multiplicands (NTT twiddles, m^-1): Shoup's method, V. Shoup, NTL
+//    "MulModPrecon": w' = floor(w * 2^64 / q); for x < 2^64,
+//    r = x*w - floor(x*w'/2^64)*q lies in [0, 2q), so one conditional
+//    subtraction gives the exact residue.  Needs q < 2^63.
+//
+// All results are the exact residues, so they are bit-identical with the
+// reference's 128-bit `%` wherever the reference itself is defined.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define HX_HD __host__ __device__ __forceinline__
+#else
+#define HX_HD inline
+#endif
+
+namespace hx {
+
+typedef unsigned __int128 u128;
+
+HX_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((u128)a * b) >> 64);
+#endif
+}
+
+HX_HD int clz64(uint64_t x) {  // x != 0
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clzll((long long)x);
+#else
+    return __builtin_clzll(x);
+#endif
+}
+
+// Reduction data for one modulus M (1 <= M < 2^64): d = M << s has its top bit
+// set and v is the Möller–Granlund reciprocal of d.
+struct ModInfo {
+    uint64_t M;
+    uint64_t d;
+    uint64_t v;
+    uint32_t s;
+    uint32_t pad;
+};
+
+// floor(((2^64-1-d) * 2^64 + 2^64-1) / d) for normalised d, by restoring
+// long division (64 steps).  Used once per modulus.
+HX_HD uint64_t mg_reciprocal(uint64_t d) {
+    uint64_t r = ~d;
+    uint64_t q = 0;
+    for (int i = 63; i >= 0; --i) {
+        uint64_t carry = r >> 63;
+        r = (r << 1) | 1u;  // the low word of the numerator is all ones
+        q <<= 1;
+        if (carry || r >= d) {
+            r -= d;
+            q |= 1u;
+        }
+    }
+    return q;
+}
+
+HX_HD ModInfo make_modinfo(uint64_t M) {
+    ModInfo mi;
+    mi.M = M;
+    mi.s = (uint32_t)clz64(M);
+    mi.d = M << mi.s;
+    mi.v = mg_reciprocal(mi.d);
+    mi.pad = 0;
+    return mi;
+}
+
+// Remainder of (u1:u0) / d for normalised d with reciprocal v; requires u1 < d.
+// Also returns the quotient word through *qout when non-null.
+HX_HD uint64_t mg_divrem(uint64_t u1, uint64_t u0, uint64_t d, uint64_t v, uint64_t* qout) {
+    u128 q = (u128)v * u1;
+    q += ((u128)(u1 + 1) << 64) | u0;
+    uint64_t q1 = (uint64_t)(q >> 64);
+    uint64_t q0 = (uint64_t)q;
+    uint64_t r = u0 - q1 * d;
+    if (r > q0) {
+        q1 -= 1;
+        r += d;
+    }
+    if (r >= d) {
+        q1 += 1;
+        r -= d;
+    }
+    if (qout) *qout = q1;
+    return r;
+}
+
+// (hi:lo) mod M; requires (hi:lo) < M * 2^64 (always true for hi < M).
+HX_HD uint64_t reduce128(const ModInfo& mi, uint64_t hi, uint64_t lo) {
+    uint32_t s = mi.s;
+    uint64_t u1 = s ? ((hi << s) | (lo >> (64 - s))) : hi;
+    uint64_t u0 = lo << s;
+    return mg_divrem(u1, u0, mi.d, mi.v, nullptr) >> s;
+}
+
+HX_HD uint64_t reduce64(const ModInfo& mi, uint64_t x) {
+    if (x < mi.M) return x;
+    return reduce128(mi, 0, x);
+}
+
+// a * b mod M for a, b < M.
+HX_HD uint64_t mulmod(const ModInfo& mi, uint64_t a, uint64_t b) {
+    u128 p = (u128)a * b;
+    return reduce128(mi, (uint64_t)(p >> 64), (uint64_t)p);
+}
+
+// floor((hi:lo) / M); requires hi < M.
+HX_HD uint64_t divq128(const ModInfo& mi, uint64_t hi, uint64_t lo) {
+    uint32_t s = mi.s;
+    uint64_t u1 = s ? ((hi << s) | (lo >> (64 - s))) : hi;
+    uint64_t u0 = lo << s;
+    uint64_t q1;
+    mg_divrem(u1, u0, mi.d, mi.v, &q1);
+    return q1;
+}
+
+// Shoup companion floor(w * 2^64 / q) for w < q.
+HX_HD uint64_t shoup_companion(const ModInfo& mi, uint64_t w) {
+    uint32_t s = mi.s;
+    uint64_t q1;
+    // (w << s) : 0 divided by (q << s) equals floor(w * 2^64 / q); w < q keeps the quotient one word.
+    mg_divrem(w << s, 0, mi.d, mi.v, &q1);
+    return q1;
+}
+
+// x * w mod q in [0, 2q) for any x < 2^64, w < q < 2^63, wp = shoup_companion(w).
+HX_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
+    uint64_t qh = mulhi64(x, wp);
+    return x * w - qh * q;
+}
+
+HX_HD uint64_t shoup_mul(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
+    uint64_t r = shoup_lazy(x, w, wp, q);
+    return r >= q ? r - q : r;
+}
+
+// 32-bit Shoup for q < 2^31: wp = floor(w * 2^32 / q).
+HX_HD uint32_t shoup_mul32(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t qh = __umulhi(x, wp);
+#else
+    uint32_t qh = (uint32_t)(((uint64_t)x * wp) >> 32);
+#endif
+    uint32_t r = x * w - qh * q;
+    return r >= q ? r - q : r;
+}
+
+HX_HD uint64_t addmod(uint64_t a, uint64_t b, uint64_t q) {  // a, b < q < 2^63
+    uint64_t s = a + b;
+    return s >= q ? s - q : s;
+}
+
+HX_HD uint64_t submod(uint64_t a, uint64_t b, uint64_t q) {  // a, b < q
+    return a >= b ? a - b : a + q - b;
+}
+
+// ---------------------------------------------------------------------------
+// C++ integer semantics of the reference's elementwise ops.
+//
+// The reference evaluates `static_cast<T>(T_DP(a) op T_DP(b)) % T_DP(p))`
+// (example_impl/modop_impl.cpp:123-126): the double-width value is exact and
+// `%` truncates toward zero, so the result is sign(x) * (|x| mod |p|).
+
+HX_HD uint64_t uabs64(int64_t x) { return x < 0 ? (uint64_t)0 - (uint64_t)x : (uint64_t)x; }
+
+HX_HD int64_t apply_sign(bool neg, uint64_t mag) { return neg ? (int64_t)((uint64_t)0 - mag) : (int64_t)mag; }
+
+// (T_DP(a) * T_DP(b)) % T_DP(p) with |p| described by mi (mi.M = |p| > 0).
+HX_HD int64_t cpp_mulmod(const ModInfo& mi, int64_t a, int64_t b) {
+    bool neg = (a < 0) != (b < 0);
+    uint64_t ma = reduce64(mi, uabs64(a));
+    uint64_t mb = reduce64(mi, uabs64(b));
+    uint64_t r = mulmod(mi, ma, mb);
+    return apply_sign(neg && r != 0, r);
+}
+
+// (T_DP(a) + T_DP(b)) % T_DP(p).
+HX_HD int64_t cpp_addmod(const ModInfo& mi, int64_t a, int64_t b) {
+    bool na = a < 0, nb = b < 0;
+    if (na == nb) {
+        uint64_t ra = reduce64(mi, uabs64(a));
+        uint64_t rb = reduce64(mi, uabs64(b));
+        uint64_t r = ra + rb;  // < 2M, may wrap only if M > 2^63
+        if (r < ra || r >= mi.M) r -= mi.M;
+        return apply_sign(na && r != 0, r);
+    }
+    int64_t s = a + b;  // opposite signs: exact in 64 bits
+    uint64_t r = reduce64(mi, uabs64(s));
+    return apply_sign(s < 0 && r != 0, r);
+}
+
+// a % b (C++ truncation) for |b| described by mi.
+HX_HD int64_t cpp_rem(const ModInfo& mi, int64_t a) {
+    uint64_t r = reduce64(mi, uabs64(a));
+    return apply_sign(a < 0 && r != 0, r);
+}
+
+}  // namespace hx

@@ -1,0 +1,622 @@
+// hx_elementwise.hip — modmul/modsum/mod, axis_modsum, g_decomposition and
+// permute for gfx950.
+//
+// All four are HBM-bound scans: one grid-stride loop per op over the
+// coalesced iteration space of the output, 64-bit element offsets decoded
+// with 32-bit fast division, and exact modular arithmetic from hx_arith.hpp.
+// A modulus that varies only along the last axis (the RNS limb axis of HEAL
+// tensors) is turned into a table of Möller–Granlund reciprocals in LDS once
+// per workgroup, so the per-element cost is one double-width product and one
+// reduction.
+#include <cstring>
+
+#include "hx_internal.hpp"
+
+namespace hx {
+
+void strided_copy(const hx_view* src, const hx_view* dst, hipStream_t st);
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTableMax = 512;
+
+// How an operand supplies values to an elementwise kernel.
+enum : int { SRC_SCALAR = 0, SRC_TABLE = 1, SRC_TENSOR = 2 };
+
+struct ElemArgs {
+    IterSpace it;          // operand order: 0 = result, 1 = a, 2 = b, 3 = p
+    void* out;
+    const void* a;
+    const void* b;
+    const void* p;
+    int64_t a_scalar;
+    int64_t b_scalar;
+    int64_t p_scalar;
+    int64_t p_table_stride;  // element stride of p along the last iteration dim (SRC_TABLE)
+    int32_t p_table_len;
+};
+
+template <typename T>
+__device__ __forceinline__ int64_t ld(const void* base, int64_t off) {
+    return (int64_t)((const T*)base)[off];
+}
+
+template <typename T>
+__device__ __forceinline__ void st_(void* base, int64_t off, int64_t v) {
+    ((T*)base)[off] = (T)v;
+}
+
+template <typename T>
+__device__ __forceinline__ int64_t to_t(int64_t v) {
+    return (int64_t)(T)v;
+}
+
+// Modulus description of a signed value: |m| with reciprocal; M == 0 marks a
+// zero modulus, whose result is defined here as 0.
+__device__ __forceinline__ ModInfo modinfo_of(int64_t m) {
+    uint64_t M = uabs64(m);
+    if (M == 0) {
+        ModInfo z;
+        z.M = 0;
+        z.d = 0;
+        z.v = 0;
+        z.s = 0;
+        z.pad = 0;
+        return z;
+    }
+    return make_modinfo(M);
+}
+
+// OP: 0 = modmul, 1 = modsum, 2 = remainder (a % b: the modulus is b, p unused)
+template <typename T, int OP, int BSRC, int PSRC>
+__global__ __launch_bounds__(kBlock) void k_modop(ElemArgs A) {
+    __shared__ ModInfo table[kTableMax];
+    // modulus operand: p for OP 0/1, b for OP 2
+    constexpr int MSRC = (OP == 2) ? BSRC : PSRC;
+    ModInfo mconst;
+    if (MSRC == SRC_SCALAR) mconst = modinfo_of(to_t<T>(OP == 2 ? A.b_scalar : A.p_scalar));
+    if (MSRC == SRC_TABLE) {
+        const void* mp = (OP == 2) ? A.b : A.p;
+        for (int i = threadIdx.x; i < A.p_table_len; i += blockDim.x)
+            table[i] = modinfo_of(ld<T>(mp, (int64_t)i * A.p_table_stride));
+        __syncthreads();
+    }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        int64_t off[4], last = 0;
+        iter_offsets<4>(A.it, idx, off, &last);
+        int64_t av = (OP == 2 && A.a == nullptr) ? to_t<T>(A.a_scalar) : ld<T>(A.a, off[1]);
+        ModInfo mi;
+        if (MSRC == SRC_SCALAR) mi = mconst;
+        else if (MSRC == SRC_TABLE) mi = table[last];
+        else mi = modinfo_of(ld<T>(OP == 2 ? A.b : A.p, OP == 2 ? off[2] : off[3]));
+        int64_t r = 0;
+        if (mi.M != 0) {
+            if (OP == 2) {
+                r = cpp_rem(mi, av);
+            } else {
+                int64_t bv = (BSRC == SRC_SCALAR) ? to_t<T>(A.b_scalar) : ld<T>(A.b, off[2]);
+                r = (OP == 0) ? cpp_mulmod(mi, av, bv) : cpp_addmod(mi, av, bv);
+            }
+        }
+        st_<T>(A.out, off[0], r);
+    }
+}
+
+// Operand strides broadcast (right-aligned) onto the result's dims: size-1
+// dims and missing leading dims get stride 0.  Operand dims larger than the
+// result's (allowed for p by the reference's checks) index by the result's
+// coordinate; smaller ones are out of range.
+void broadcast_strides(const hx_view* v, const hx_view* r, int64_t* out, const char* name) {
+    int shift = r->ndim - v->ndim;
+    if (shift < 0) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " has more dims than result.");
+    for (int i = 0; i < r->ndim; ++i) out[i] = 0;
+    for (int i = 0; i < v->ndim; ++i) {
+        if (v->dims[i] == 1) continue;
+        if (v->dims[i] < r->dims[i + shift] && view_numel(r) > 0) raise(HX_ERR_OUT_OF_RANGE, "Index out of bounds.");
+        out[i + shift] = v->strides[i];
+    }
+}
+
+// CHECK_DIMS_BROADCASTABLE (example_impl/modop_impl.cpp:89-101)
+void check_broadcastable(const hx_view* t, const hx_view* r, const char* label) {
+    if (t->ndim > r->ndim) raise(HX_ERR_INVALID_ARGUMENT, std::string(label) + " has more dims than result.");
+    for (int i = 1; i <= t->ndim; ++i) {
+        int64_t td = t->dims[t->ndim - i], rd = r->dims[r->ndim - i];
+        if (td != 1 && td != rd)
+            raise(HX_ERR_INVALID_ARGUMENT, std::string(label) + " not broadcast-compatible with result.");
+    }
+}
+
+void require_int(const hx_view* v, const hx_view* r, const char* name) {
+    if (v->dtype != HX_INT32 && v->dtype != HX_INT64)
+        raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + ": modular ops need int32 or int64 tensors");
+    if (v->dtype != r->dtype) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + ": dtype differs from result");
+}
+
+template <typename T, int OP>
+void launch_modop_t(ElemArgs& A, int bsrc, int psrc, hipStream_t st) {
+    unsigned g = grid_for(A.it.total, kBlock);
+#define HX_LAUNCH(B, P) hipLaunchKernelGGL((k_modop<T, OP, B, P>), dim3(g), dim3(kBlock), 0, st, A)
+    if (bsrc == SRC_TENSOR) {
+        if (psrc == SRC_SCALAR) HX_LAUNCH(SRC_TENSOR, SRC_SCALAR);
+        else if (psrc == SRC_TABLE) HX_LAUNCH(SRC_TENSOR, SRC_TABLE);
+        else HX_LAUNCH(SRC_TENSOR, SRC_TENSOR);
+    } else if (bsrc == SRC_TABLE) {
+        HX_LAUNCH(SRC_TABLE, SRC_SCALAR);
+    } else {
+        if (psrc == SRC_SCALAR) HX_LAUNCH(SRC_SCALAR, SRC_SCALAR);
+        else if (psrc == SRC_TABLE) HX_LAUNCH(SRC_SCALAR, SRC_TABLE);
+        else HX_LAUNCH(SRC_SCALAR, SRC_TENSOR);
+    }
+#undef HX_LAUNCH
+    check_hip(hipGetLastError(), "modop launch");
+}
+
+// Decide whether the modulus operand (iteration operand index `mop`) varies
+// only along the last iteration dim; if so use an LDS table.
+int modulus_source(const ElemArgs& A, int mop) {
+    const IterSpace& it = A.it;
+    for (int d = 0; d < it.ndim - 1; ++d)
+        if (it.strides[mop][d] != 0) return SRC_TENSOR;
+    if (it.dims[it.ndim - 1] <= kTableMax) return SRC_TABLE;
+    return SRC_TENSOR;
+}
+
+// modmul_* / modsum_* (example_impl/modop_impl.cpp:108-180)
+void modop(int op, const hx_view* a, const hx_view* b, int64_t bs, const hx_view* p, int64_t ps,
+           const hx_view* result, hipStream_t st) {
+    check_view(a, "a");
+    check_view(result, "result");
+    if (b) check_view(b, "b");
+    if (p) check_view(p, "p");
+    require_int(result, result, "result");
+    require_int(a, result, "a");
+    if (b) require_int(b, result, "b");
+    if (p) require_int(p, result, "p");
+    check_broadcastable(a, result, "a");
+    if (b) check_broadcastable(b, result, "b");
+    if (p) {
+        // CHECK_DIMS_MATCH_LAST (example_impl/modop_impl.cpp:84-87)
+        if (p->ndim != 1 && (p->ndim == 0 || result->ndim == 0 || p->dims[p->ndim - 1] != result->dims[result->ndim - 1]))
+            raise(HX_ERR_INVALID_ARGUMENT, "Last dimension of p must match last dimension of result.");
+    }
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0; i < result->ndim; ++i) strides[0][i] = result->strides[i];
+    broadcast_strides(a, result, strides[1], "a");
+    if (b) broadcast_strides(b, result, strides[2], "b");
+    if (p) broadcast_strides(p, result, strides[3], "p");
+    ElemArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(result->ndim, result->dims, 4, strides, p != nullptr);
+    if (A.it.total == 0) return;
+    A.out = result->data;
+    A.a = a->data;
+    A.b = b ? b->data : nullptr;
+    A.p = p ? p->data : nullptr;
+    A.b_scalar = bs;
+    A.p_scalar = ps;
+    int bsrc = b ? SRC_TENSOR : SRC_SCALAR;
+    int psrc = SRC_SCALAR;
+    if (p) {
+        psrc = modulus_source(A, 3);
+        if (psrc == SRC_TABLE) {
+            A.p_table_stride = A.it.strides[3][A.it.ndim - 1];
+            A.p_table_len = (int32_t)A.it.dims[A.it.ndim - 1];
+        }
+    }
+    bool is32 = result->dtype == HX_INT32;
+    if (op == 0) {
+        if (is32) launch_modop_t<int32_t, 0>(A, bsrc, psrc, st);
+        else launch_modop_t<int64_t, 0>(A, bsrc, psrc, st);
+    } else {
+        if (is32) launch_modop_t<int32_t, 1>(A, bsrc, psrc, st);
+        else launch_modop_t<int64_t, 1>(A, bsrc, psrc, st);
+    }
+}
+
+// mod_tt / mod_tc / mod_ct (example_impl/modop_impl.cpp:183-235)
+void modrem(const hx_view* a, bool a_tensor, int64_t as, const hx_view* b, bool b_tensor, int64_t bs,
+            const hx_view* result, hipStream_t st) {
+    if (a_tensor && !a) raise(HX_ERR_INVALID_ARGUMENT, "a pointer must not be null.");
+    if (b_tensor && !b) raise(HX_ERR_INVALID_ARGUMENT, "b pointer must not be null.");
+    check_view(result, "result");
+    require_int(result, result, "result");
+    if (a) {
+        check_view(a, "a");
+        require_int(a, result, "a");
+        if (!dims_equal(a, result)) raise(HX_ERR_INVALID_ARGUMENT, "a must have exactly the same shape as result.");
+    }
+    if (b) {
+        check_view(b, "b");
+        require_int(b, result, "b");
+        if (!dims_equal(b, result)) raise(HX_ERR_INVALID_ARGUMENT, "b must have exactly the same shape as result.");
+    }
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0; i < result->ndim; ++i) {
+        strides[0][i] = result->strides[i];
+        if (a) strides[1][i] = a->strides[i];
+        if (b) strides[2][i] = b->strides[i];
+    }
+    ElemArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(result->ndim, result->dims, 3, strides, false);
+    if (A.it.total == 0) return;
+    A.out = result->data;
+    A.a = a ? a->data : nullptr;
+    A.b = b ? b->data : nullptr;
+    A.a_scalar = as;
+    A.b_scalar = bs;
+    int bsrc = b ? SRC_TENSOR : SRC_SCALAR;
+    if (result->dtype == HX_INT32) launch_modop_t<int32_t, 2>(A, bsrc, SRC_SCALAR, st);
+    else launch_modop_t<int64_t, 2>(A, bsrc, SRC_SCALAR, st);
+}
+
+// ------------------------------------------------------------ axis_modsum
+struct AxisArgs {
+    IterSpace it;  // over result: operand 0 = result, 1 = a (axis removed)
+    void* out;
+    const void* a;
+    const void* p;
+    int64_t axis_len;
+    int64_t axis_stride;   // stride of a along the reduced axis
+    int64_t p_stride;      // stride of p (1-D)
+    int32_t table;         // 1: p indexed by the last coordinate from LDS
+    int32_t k;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axis_modsum(AxisArgs A) {
+    __shared__ ModInfo table[kTableMax];
+    if (A.table) {
+        for (int i = threadIdx.x; i < A.k; i += blockDim.x) table[i] = modinfo_of(ld<T>(A.p, (int64_t)i * A.p_stride));
+        __syncthreads();
+    }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        int64_t off[2], last = 0;
+        iter_offsets<2>(A.it, idx, off, &last);
+        ModInfo mi = A.table ? table[last] : modinfo_of(ld<T>(A.p, last * A.p_stride));
+        int64_t sum = 0;
+        const int64_t base = off[1];
+        for (int64_t r = 0; r < A.axis_len; ++r) {
+            int64_t s = (int64_t)((uint64_t)sum + (uint64_t)ld<T>(A.a, base + r * A.axis_stride));
+            int64_t red;
+            if (mi.M == 0) red = 0;
+            else if (s >= 0 && (uint64_t)s < mi.M) red = s;
+            else if (s >= 0 && mi.M <= (1ull << 62) && (uint64_t)s < 2 * mi.M) red = s - (int64_t)mi.M;
+            else red = cpp_rem(mi, s);
+            sum = to_t<T>(red);
+        }
+        st_<T>(A.out, off[0], sum);
+    }
+}
+
+// axis_modsum (example_impl/axis_modsum_impl.cpp:12-76)
+void axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, int64_t axis, hipStream_t st) {
+    check_view(a, "a");
+    check_view(p, "p");
+    check_view(result, "result");
+    require_int(result, result, "result");
+    require_int(a, result, "a");
+    require_int(p, result, "p");
+    if (p->ndim != 1) raise(HX_ERR_INVALID_ARGUMENT, "p must be a 1D tensor of shape [k]");
+    const int nd = a->ndim;
+    if (axis < 0 || axis >= nd - 1)
+        raise(HX_ERR_INVALID_ARGUMENT, "axis must be in range [0, ndim - 2] (can't reduce across last axis)");
+    if (a->dims[nd - 1] != p->dims[0]) raise(HX_ERR_INVALID_ARGUMENT, "Last dimension of a must match shape of p");
+    if (result->ndim != nd - 1) raise(HX_ERR_INVALID_ARGUMENT, "result must have the shape of a without axis");
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0, j = 0; i < nd; ++i) {
+        if (i == axis) continue;
+        if (result->dims[j] != a->dims[i]) raise(HX_ERR_INVALID_ARGUMENT, "result must have the shape of a without axis");
+        strides[0][j] = result->strides[j];
+        strides[1][j] = a->strides[i];
+        ++j;
+    }
+    AxisArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(result->ndim, result->dims, 2, strides, true);
+    if (A.it.total == 0) return;
+    A.out = result->data;
+    A.a = a->data;
+    A.p = p->data;
+    A.axis_len = a->dims[axis];
+    A.axis_stride = a->strides[axis];
+    A.p_stride = p->strides[0];
+    A.k = (int32_t)p->dims[0];
+    A.table = p->dims[0] <= kTableMax;
+    unsigned g = grid_for(A.it.total, kBlock);
+    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_axis_modsum<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_axis_modsum<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    check_hip(hipGetLastError(), "axis_modsum launch");
+}
+
+// -------------------------------------------------------- g_decomposition
+struct GdArgs {
+    IterSpace it;  // over a: operand 0 = result (digit 0), 1 = a
+    void* out;
+    const void* a;
+    int64_t digit_stride;
+    uint32_t power;
+    uint32_t base_bits;
+    DevStatus* status;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gdecomp(GdArgs A) {
+    const uint64_t mask = (A.base_bits == 0) ? 0 : ((~0ull) >> (64 - A.base_bits));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long over = 0;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        int64_t off[2], last;
+        iter_offsets<2>(A.it, idx, off, &last);
+        int64_t v = ld<T>(A.a, off[1]);
+        for (uint32_t d = 0; d < A.power; ++d) {
+            uint64_t u = (uint64_t)v;  // size_t arithmetic of the reference
+            st_<T>(A.out, off[0] + (int64_t)d * A.digit_stride, to_t<T>((int64_t)(u & mask)));
+            v = to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u));
+        }
+        if (v > 0) ++over;
+    }
+    if (over) atomicAdd(&A.status->gdecomp_overflow, over);
+}
+
+// g_decomposition (example_impl/g_decomposition_impl.cpp:10-67)
+void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hipStream_t st) {
+    check_view(a, "a");
+    check_view(result, "result");
+    require_int(result, result, "result");
+    require_int(a, result, "a");
+    bool ok = result->ndim == a->ndim + 1 && result->dims[result->ndim - 1] == (int64_t)power;
+    for (int i = 0; ok && i < a->ndim; ++i) ok = result->dims[i] == a->dims[i];
+    if (!ok) raise(HX_ERR_INVALID_ARGUMENT, "Output must have shape a.shape + [power]");
+    if (base_bits >= 64) raise(HX_ERR_INVALID_ARGUMENT, "base_bits must be < 64");
+    if (power > 0xffffffffull) raise(HX_ERR_INVALID_ARGUMENT, "power too large");
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0; i < a->ndim; ++i) {
+        strides[0][i] = result->strides[i];
+        strides[1][i] = a->strides[i];
+    }
+    GdArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(a->ndim, a->dims, 2, strides, false);
+    if (A.it.total == 0) return;
+    A.out = result->data;
+    A.a = a->data;
+    A.digit_stride = result->strides[result->ndim - 1];
+    A.power = (uint32_t)power;
+    A.base_bits = (uint32_t)base_bits;
+    A.status = device_status();
+    unsigned g = grid_for(A.it.total, kBlock);
+    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_gdecomp<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_gdecomp<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    check_hip(hipGetLastError(), "g_decomposition launch");
+}
+
+// ------------------------------------------------------------------ permute
+struct PermArgs {
+    IterSpace it;  // over result dims (no merging across the two axes): 0 = result, 1 = a
+    void* out;
+    const void* a;
+    const void* perms;
+    int32_t ew_dim;   // iteration dim index of elementwise_axis
+    int32_t pa_dim;   // iteration dim index of perm_axis
+    int64_t a_pa_stride;
+    int64_t ps0, ps1;  // perms strides
+    int64_t m;
+    DevStatus* status;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute(PermArgs A) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        // full decode (the two permutation axes are kept as separate dims)
+        int64_t ooff = 0, aoff = 0, cew = 0, cpa = 0;
+        int64_t n = idx;
+        for (int d = A.it.ndim - 1; d >= 0; --d) {
+            int64_t c;
+            if (A.it.use32) {
+                uint32_t q = fdiv((uint32_t)n, A.it.fd[d]);
+                c = (uint32_t)n - q * A.it.fd[d].d;
+                n = q;
+            } else {
+                c = n % A.it.dims[d];
+                n /= A.it.dims[d];
+            }
+            if (d == A.ew_dim) cew = c;
+            if (d == A.pa_dim) cpa = c;
+            ooff += c * A.it.strides[0][d];
+            aoff += c * A.it.strides[1][d];
+        }
+        int64_t pi = ld<T>(A.perms, cew * A.ps0 + cpa * A.ps1);
+        int64_t v = 0;
+        if (pi >= 0 && pi < A.m) v = ld<T>(A.a, aoff + (pi - cpa) * A.a_pa_stride);
+        else ++bad;
+        st_<T>(A.out, ooff, v);
+    }
+    if (bad) atomicAdd(&A.status->index_out_of_range, bad);
+}
+
+}  // namespace
+
+// permute (example_impl/permute_impl.cpp:8-65)
+void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int64_t ew, int64_t pa, bool trusted,
+             hipStream_t st) {
+    check_view(a, "a");
+    check_view(perms, "perms");
+    check_view(result, "result");
+    require_int(result, result, "result");
+    require_int(a, result, "a");
+    require_int(perms, result, "perms");
+    const int nd = a->ndim;
+    if (!dims_equal(result, a)) raise(HX_ERR_INVALID_ARGUMENT, "Result shape must match input shape.");
+    if (ew < 0 || ew >= nd || pa < 0 || pa >= nd) raise(HX_ERR_INVALID_ARGUMENT, "Axis out of bounds.");
+    if (ew == pa) raise(HX_ERR_INVALID_ARGUMENT, "elementwise_axis and perm_axis must be different.");
+    const int64_t l = a->dims[ew], m = a->dims[pa];
+    if (perms->ndim != 2 || perms->dims[0] != l || perms->dims[1] != m)
+        raise(HX_ERR_INVALID_ARGUMENT,
+              "Perms must have shape [l, m] where l and m match a.shape at elementwise and perm axes.");
+    if (view_numel(a) == 0) return;
+    // A result that overlaps the input is computed from a private copy of the input.
+    hx_view src = *a;
+    void* tmp = nullptr;
+    if (views_overlap(a, result)) {
+        size_t bytes = (size_t)view_numel(a) * dtype_size(a->dtype);
+        check_hip(hipMallocAsync(&tmp, bytes, st), "hipMallocAsync(permute tmp)");
+        src.data = tmp;
+        int64_t s = 1;
+        for (int i = nd - 1; i >= 0; --i) {
+            src.strides[i] = s;
+            s *= a->dims[i];
+        }
+        strided_copy(a, &src, st);
+    }
+    DevStatus* status = device_status();
+    if (!trusted) {
+        // count only this call's out-of-range entries
+        check_hip(hipMemsetAsync(&status->index_out_of_range, 0, sizeof(unsigned long long), st), "hipMemsetAsync");
+    }
+    // Iteration space without size-1 dropping or merging, so that the two
+    // permutation axes keep their own coordinates.
+    PermArgs A;
+    std::memset(&A, 0, sizeof A);
+    IterSpace& it = A.it;
+    it.ndim = nd;
+    it.total = view_numel(a);
+    it.use32 = it.total < (int64_t(1) << 31);
+    for (int i = 0; i < nd; ++i) {
+        it.dims[i] = a->dims[i];
+        it.fd[i] = make_fastdiv(it.use32 ? (uint32_t)a->dims[i] : 1u);
+        it.strides[0][i] = result->strides[i];
+        it.strides[1][i] = src.strides[i];
+    }
+    A.out = result->data;
+    A.a = src.data;
+    A.perms = perms->data;
+    A.ew_dim = (int32_t)ew;
+    A.pa_dim = (int32_t)pa;
+    A.a_pa_stride = src.strides[pa];
+    A.ps0 = perms->strides[0];
+    A.ps1 = perms->strides[1];
+    A.m = m;
+    A.status = status;
+    unsigned g = grid_for(it.total, kBlock);
+    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_permute<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_permute<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    check_hip(hipGetLastError(), "permute launch");
+    if (tmp) check_hip(hipFreeAsync(tmp, st), "hipFreeAsync");
+    if (!trusted) {
+        unsigned long long bad = 0;
+        check_hip(hipMemcpyAsync(&bad, &status->index_out_of_range, sizeof bad, hipMemcpyDeviceToHost, st),
+                  "hipMemcpyAsync(status)");
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (bad) {
+            check_hip(hipMemsetAsync(&status->index_out_of_range, 0, sizeof bad, st), "hipMemsetAsync");
+            check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+            raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
+        }
+    }
+}
+
+}  // namespace hx
+
+using namespace hx;
+
+extern "C" {
+
+int hx_modmul_ttt(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!b) raise(HX_ERR_INVALID_ARGUMENT, "b pointer must not be null.");
+    if (!p) raise(HX_ERR_INVALID_ARGUMENT, "p pointer must not be null.");
+    hx::modop(0, a, b, 0, p, 0, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modmul_ttc(const hx_view* a, const hx_view* b, int64_t p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!b) raise(HX_ERR_INVALID_ARGUMENT, "b pointer must not be null.");
+    hx::modop(0, a, b, 0, nullptr, p, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modmul_tct(const hx_view* a, int64_t b, const hx_view* p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!p) raise(HX_ERR_INVALID_ARGUMENT, "p pointer must not be null.");
+    hx::modop(0, a, nullptr, b, p, 0, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modmul_tcc(const hx_view* a, int64_t b, int64_t p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    hx::modop(0, a, nullptr, b, nullptr, p, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modsum_ttt(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!b) raise(HX_ERR_INVALID_ARGUMENT, "b pointer must not be null.");
+    if (!p) raise(HX_ERR_INVALID_ARGUMENT, "p pointer must not be null.");
+    hx::modop(1, a, b, 0, p, 0, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modsum_ttc(const hx_view* a, const hx_view* b, int64_t p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!b) raise(HX_ERR_INVALID_ARGUMENT, "b pointer must not be null.");
+    hx::modop(1, a, b, 0, nullptr, p, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modsum_tct(const hx_view* a, int64_t b, const hx_view* p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!p) raise(HX_ERR_INVALID_ARGUMENT, "p pointer must not be null.");
+    hx::modop(1, a, nullptr, b, p, 0, result, as_stream(stream));
+    HX_API_END
+}
+int hx_modsum_tcc(const hx_view* a, int64_t b, int64_t p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    hx::modop(1, a, nullptr, b, nullptr, p, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_mod_tt(const hx_view* a, const hx_view* b, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    hx::modrem(a, true, 0, b, true, 0, result, as_stream(stream));
+    HX_API_END
+}
+int hx_mod_tc(const hx_view* a, int64_t b, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    hx::modrem(a, true, 0, nullptr, false, b, result, as_stream(stream));
+    HX_API_END
+}
+int hx_mod_ct(int64_t a, const hx_view* b, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    hx::modrem(nullptr, false, a, b, true, 0, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, int64_t axis, hx_stream stream) {
+    HX_API_BEGIN
+    hx::axis_modsum(a, p, result, axis, as_stream(stream));
+    HX_API_END
+}
+
+int hx_g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hx_stream stream) {
+    HX_API_BEGIN
+    hx::g_decomposition(a, result, power, base_bits, as_stream(stream));
+    HX_API_END
+}
+
+int hx_permute(const hx_view* a, const hx_view* perms, const hx_view* result, int64_t elementwise_axis,
+               int64_t perm_axis, hx_stream stream) {
+    HX_API_BEGIN
+    hx::permute(a, perms, result, elementwise_axis, perm_axis, false, as_stream(stream));
+    HX_API_END
+}
+
+int hx_permute_ex(const hx_view* a, const hx_view* perms, const hx_view* result, int64_t elementwise_axis,
+                  int64_t perm_axis, uint32_t flags, hx_stream stream) {
+    HX_API_BEGIN
+    hx::permute(a, perms, result, elementwise_axis, perm_axis, (flags & HX_PERMS_TRUSTED) != 0, as_stream(stream));
+    HX_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+// hx_internal.hpp — shared host/device plumbing of the HEAL MI355X backend:
+// error reporting at the C-ABI boundary, view validation, iteration-shape
+// coalescing for strided kernels, fast 32-bit division by invariants, and the
+// per-device status block that kernels use to report data-dependent errors.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/heal_amd.h"
+#include "hx_arith.hpp"
+
+namespace hx {
+
+// ------------------------------------------------------------------ errors
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void raise(int code, const std::string& msg);
+int set_error(int code, const std::string& msg);
+void check_hip(hipError_t e, const char* what);
+
+#define HX_API_BEGIN try {
+#define HX_API_END                                                        \
+    }                                                                     \
+    catch (const hx::Error& e) {                                          \
+        return hx::set_error(e.code, e.msg);                              \
+    }                                                                     \
+    catch (const std::bad_alloc&) {                                       \
+        return hx::set_error(HX_ERR_BAD_ALLOC, "std::bad_alloc");         \
+    }                                                                     \
+    catch (const std::exception& e) {                                     \
+        return hx::set_error(HX_ERR_RUNTIME, e.what());                   \
+    }                                                                     \
+    return HX_OK;
+
+// ------------------------------------------------------------------- views
+size_t dtype_size(int dtype);
+void check_view(const hx_view* v, const char* name);  // throws on malformed descriptors
+int64_t view_numel(const hx_view* v);
+bool dims_equal(const hx_view* a, const hx_view* b);
+// [lowest, highest] byte range touched by a view (empty views give lo > hi).
+void view_byte_range(const hx_view* v, uintptr_t* lo, uintptr_t* hi);
+bool views_overlap(const hx_view* a, const hx_view* b);
+
+// ------------------------------------------- fast division by an invariant
+// Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994 (round-up variant): for n < 2^31,
+// n / d = (umulhi(n, mul) + n) >> shift.
+struct FastDiv {
+    uint32_t d;
+    uint32_t mul;
+    uint32_t shift;
+    uint32_t pad;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) ++l;
+    f.shift = l;
+    f.mul = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    f.pad = 0;
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    return (__umulhi(n, f.mul) + n) >> f.shift;
+}
+
+// ------------------------------------------------------- iteration shapes
+// A strided iteration space over up to HX_MAX_DIMS dims with up to 4 operands.
+// Size-1 dims are dropped and adjacent dims that are contiguous for every
+// operand are merged, so kernels decode as few coordinates as possible.
+constexpr int kMaxOps = 4;
+
+struct IterSpace {
+    int32_t ndim;
+    int32_t use32;  // total < 2^31: 32-bit fast division
+    int64_t total;
+    int64_t dims[HX_MAX_DIMS];
+    FastDiv fd[HX_MAX_DIMS];
+    int64_t strides[kMaxOps][HX_MAX_DIMS];
+};
+
+// dims: iteration dims (ndim entries); strides[o][i]: stride of operand o along
+// dim i (0 for broadcast).  keep_last: never merge the last dim away (used when
+// a kernel indexes a table by the last coordinate).
+IterSpace make_iter(int ndim, const int64_t* dims, int nops, const int64_t (*strides)[HX_MAX_DIMS], bool keep_last);
+
+// Decode a linear index into per-operand element offsets; also returns the
+// coordinate along the last iteration dim.
+template <int NOPS>
+__device__ __forceinline__ void iter_offsets(const IterSpace& it, int64_t idx, int64_t* off, int64_t* last) {
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) off[o] = 0;
+    if (it.use32) {
+        uint32_t n = (uint32_t)idx;
+        for (int d = it.ndim - 1; d >= 0; --d) {
+            uint32_t q = fdiv(n, it.fd[d]);
+            uint32_t c = n - q * it.fd[d].d;
+            if (d == it.ndim - 1) *last = c;
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) off[o] += (int64_t)c * it.strides[o][d];
+            n = q;
+        }
+    } else {
+        for (int d = it.ndim - 1; d >= 0; --d) {
+            int64_t c = idx % it.dims[d];
+            if (d == it.ndim - 1) *last = c;
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) off[o] += c * it.strides[o][d];
+            idx /= it.dims[d];
+        }
+    }
+}
+
+// ------------------------------------------------------- device status
+struct DevStatus {
+    unsigned long long gdecomp_overflow;
+    unsigned long long index_out_of_range;
+    unsigned long long spare[6];
+};
+DevStatus* device_status();  // current device's block (allocated on first use)
+
+// grid size for grid-stride kernels
+inline unsigned grid_for(int64_t total, int block, int64_t cap = 4096) {
+    int64_t g = (total + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+inline hipStream_t as_stream(hx_stream s) { return (hipStream_t)s; }
+
+}  // namespace hx

@@ -1,0 +1,160 @@
+/*
+ * heal_amd.h — C ABI of the MI355X (gfx950) HEAL backend.
+ *
+ * Every compute entry point of the HEAL hardware API (reference: the
+ * include/ headers of Lattica-ai/heal) has one extern "C" function here.  Tensors cross the boundary
+ * as plain view descriptors (device pointer, dtype, dims, strides in elements);
+ * no C++ or torch type appears in any signature.  The C++17 templates of the HEAL
+ * API (namespace lattica_hw_api) and the pybind11 module `lattica_hw` are built on
+ * top of this layer.
+ *
+ * Conventions
+ *  - All functions return an hx_status.  On failure hx_last_error() gives the
+ *    message (thread-local).  HX_ERR_INVALID_ARGUMENT / HX_ERR_OUT_OF_RANGE /
+ *    HX_ERR_RUNTIME / HX_ERR_BAD_ALLOC correspond to std::invalid_argument,
+ *    std::out_of_range, std::runtime_error and std::bad_alloc of the reference.
+ *  - Work is enqueued on `stream` (a hipStream_t; NULL = the legacy default
+ *    stream) and is asynchronous unless the description says otherwise.
+ *  - A view's `data` points at element [0,...,0]; strides may be 0 (broadcast).
+ *  - Element types: HX_INT32, HX_INT64 for compute ops; HX_FLOAT64 only for the
+ *    memory/view ops, as in the reference.
+ *  - Results are exact integers.  Inputs of the modular ops follow C++ integer
+ *    semantics of the reference (truncating remainder, 128-bit/64-bit
+ *    intermediates).  NTT/INTT inputs must be non-negative; outputs are the
+ *    canonical residues in [0, q).
+ */
+#ifndef HEAL_AMD_H
+#define HEAL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HX_MAX_DIMS 8
+#define HX_ABI_VERSION 1
+
+typedef enum hx_dtype { HX_INT32 = 0, HX_INT64 = 1, HX_FLOAT64 = 2 } hx_dtype;
+
+typedef enum hx_status {
+    HX_OK = 0,
+    HX_ERR_INVALID_ARGUMENT = 1,
+    HX_ERR_OUT_OF_RANGE = 2,
+    HX_ERR_RUNTIME = 3,
+    HX_ERR_BAD_ALLOC = 4
+} hx_status;
+
+typedef struct hx_view {
+    void*   data;                  /* device address of element [0,...,0] */
+    int32_t dtype;                 /* hx_dtype */
+    int32_t ndim;                  /* 0..HX_MAX_DIMS */
+    int64_t dims[HX_MAX_DIMS];
+    int64_t strides[HX_MAX_DIMS];  /* in elements */
+} hx_view;
+
+typedef void* hx_stream;           /* hipStream_t */
+
+/* Sticky per-device counters written by kernels for data-dependent conditions. */
+typedef struct hx_device_status {
+    uint64_t gdecomp_overflow;     /* g_decomposition elements whose value was not exhausted */
+    uint64_t index_out_of_range;   /* permute / NTT perm entries outside [0, m) */
+} hx_device_status;
+
+/* ------------------------------------------------------------------ library */
+int         hx_abi_version(void);
+const char* hx_last_error(void);
+/* Synchronises `stream`, copies the current device's status counters into *out
+ * (may be NULL) and clears them. */
+int hx_status_poll(hx_stream stream, hx_device_status* out);
+
+/* ---------------------------------------------------- memory (L0 of the API)
+ * Replaces the storage side of DeviceTensor / allocate_on_hardware /
+ * host_to_device / device_to_host
+ * (reference include/device_memory.h:24-39, example_impl/device_memory_impl.cpp:168-202).
+ * Allocation is stream-ordered from a caching pool. */
+int hx_malloc(void** out, size_t bytes, hx_stream stream);
+int hx_free(void* ptr, hx_stream stream);
+int hx_memset_zero(void* ptr, size_t bytes, hx_stream stream);          /* fill kernel, graph-safe */
+int hx_memcpy_h2d(void* dst, const void* src, size_t bytes, hx_stream stream); /* returns when src may be reused */
+int hx_memcpy_d2h(void* dst, const void* src, size_t bytes, hx_stream stream); /* returns when dst is filled */
+int hx_memcpy_d2d(void* dst, const void* src, size_t bytes, hx_stream stream);
+int hx_stream_sync(hx_stream stream);
+
+/* ------------------------------------------------------------ views (L1)
+ * make_contiguous (reference include/contiguous.h:19-20, example_impl/contiguous_impl.cpp:9-52):
+ * gathers the strided view `src` into `dst` (same dims, any strides, normally
+ * C-contiguous).  Any dtype. */
+int hx_strided_copy(const hx_view* src, const hx_view* dst, hx_stream stream);
+
+/* ------------------------------------------------- modular elementwise (L2)
+ * result = (a (*|+) b) % p with right-aligned broadcasting of a, b, p to result
+ * (reference include/modop.h:47-111, example_impl/modop_impl.cpp:108-180).
+ * Scalars are passed as int64 and converted to the element type. */
+int hx_modmul_ttt(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, hx_stream stream);
+int hx_modmul_ttc(const hx_view* a, const hx_view* b, int64_t p, const hx_view* result, hx_stream stream);
+int hx_modmul_tct(const hx_view* a, int64_t b, const hx_view* p, const hx_view* result, hx_stream stream);
+int hx_modmul_tcc(const hx_view* a, int64_t b, int64_t p, const hx_view* result, hx_stream stream);
+int hx_modsum_ttt(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, hx_stream stream);
+int hx_modsum_ttc(const hx_view* a, const hx_view* b, int64_t p, const hx_view* result, hx_stream stream);
+int hx_modsum_tct(const hx_view* a, int64_t b, const hx_view* p, const hx_view* result, hx_stream stream);
+int hx_modsum_tcc(const hx_view* a, int64_t b, int64_t p, const hx_view* result, hx_stream stream);
+
+/* result = a % b, C++ truncating remainder, same shapes only
+ * (reference include/modop.h:115-134, example_impl/modop_impl.cpp:183-235). */
+int hx_mod_tt(const hx_view* a, const hx_view* b, const hx_view* result, hx_stream stream);
+int hx_mod_tc(const hx_view* a, int64_t b, const hx_view* result, hx_stream stream);
+int hx_mod_ct(int64_t a, const hx_view* b, const hx_view* result, hx_stream stream);
+
+/* result = sum over `axis` of a, reduced mod p[k] after every term
+ * (reference include/axis_modsum.h:24-30, example_impl/axis_modsum_impl.cpp:12-76). */
+int hx_axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, int64_t axis, hx_stream stream);
+
+/* result[..., d] = d-th base-2^base_bits digit of a (unsigned digits)
+ * (reference include/g_decomposition.h:24-30, example_impl/g_decomposition_impl.cpp:10-67).
+ * Elements whose value is not exhausted after `power` digits increment the
+ * gdecomp_overflow counter (the reference prints a warning). */
+int hx_g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hx_stream stream);
+
+/* Negacyclic NTT / INTT over axis 1 of [l, m, r, k] views
+ * (reference include/ntt.h:31-52, example_impl/ntt_impl.cpp:69-170).
+ * log2p_list and mu_list may be NULL and are not read (as in the reference).
+ * These build a temporary plan (see hx_ntt_plan_*) and synchronise the stream
+ * once to classify `perm`. */
+int hx_ntt(const hx_view* a, const hx_view* p, const hx_view* perm, const hx_view* twiddles,
+           const hx_view* log2p_list, const hx_view* mu_list, const hx_view* result, hx_stream stream);
+int hx_intt(const hx_view* a, const hx_view* p, const hx_view* perm, const hx_view* inv_twiddles,
+            const hx_view* m_inv, const hx_view* log2p_list, const hx_view* mu_list,
+            const hx_view* result, hx_stream stream);
+
+/* result[c] = a[c with c[perm_axis] := perms[c[elementwise_axis], c[perm_axis]]]
+ * (reference include/permute.h:33-40, example_impl/permute_impl.cpp:8-65).
+ * Synchronises the stream to report out-of-range perms as HX_ERR_OUT_OF_RANGE. */
+int hx_permute(const hx_view* a, const hx_view* perms, const hx_view* result,
+               int64_t elementwise_axis, int64_t perm_axis, hx_stream stream);
+
+/* ------------------------------------------------------ plans and batching
+ * No counterpart in the reference: they let callers that replay the same
+ * constants (twiddle tables, Galois maps) skip per-call preparation and host
+ * synchronisation, which the HEAL API's per-call form cannot express.
+ *
+ * An NTT plan is an immutable device snapshot of (p, perm, twiddles[, m_inv])
+ * with Shoup companions precomputed; executing it never reads the source
+ * tensors again and never synchronises. */
+typedef struct hx_ntt_plan hx_ntt_plan;
+int hx_ntt_plan_create(hx_ntt_plan** out, int inverse, const hx_view* p, const hx_view* perm,
+                       const hx_view* twiddles, const hx_view* m_inv, hx_stream stream);
+int hx_ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream);
+int hx_ntt_plan_destroy(hx_ntt_plan* plan);
+/* permute without the host synchronisation: perms must have been validated
+ * (e.g. by an earlier hx_permute); out-of-range entries still never read
+ * outside `a`, they write 0 and count in index_out_of_range. */
+#define HX_PERMS_TRUSTED 1u
+int hx_permute_ex(const hx_view* a, const hx_view* perms, const hx_view* result,
+                  int64_t elementwise_axis, int64_t perm_axis, uint32_t flags, hx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEAL_AMD_H */
