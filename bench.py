@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (scales HBM byte counters under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -106,35 +108,57 @@ def main():
     from heal_amd.keyswitch import KeySwitchBatch
 
     torch.cuda.init()
+    if args.copy_calibration:
+        src = torch.ones(32 << 20, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     N, L, B = args.n, args.limbs, args.batch
     batched = KeySwitchBatch(n=N, limbs=L, batch=B, seed=0)
     plain = KeySwitchBatch(n=N, limbs=L, batch=1, seed=0, primes=batched.primes)
     plain.x.copy_(batched.x[:1])
     plain.evk.copy_(batched.evk[:, :1])
 
-    # correctness gate: plain == batched eager == graph replay, bit for bit
+    # correctness gate, bit for bit: plain chain == batched chain == chain graph
+    # == fused kernels == fused graph
     plain.run()
     batched.run()
     torch.cuda.synchronize()
-    ref0 = batched.out[0].clone()
-    assert torch.equal(plain.out[0], ref0), "plain and batched forms differ"
-    batched.capture()
-    batched.out.zero_()
-    batched.replay()
+    ref_all = batched.out.clone()
+    assert torch.equal(plain.out[0], ref_all[0]), "plain and batched forms differ"
+    fused_out = torch.zeros_like(batched.out)
+    batched.run_fused(fused_out)
     torch.cuda.synchronize()
-    assert torch.equal(batched.out[0], ref0), "graph replay differs from eager"
+    assert torch.equal(fused_out, ref_all), "fused form differs from the six-call chain"
+    chain_graph = batched.capture(fused=False)
+    batched.out.zero_()
+    chain_graph.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(batched.out, ref_all), "chain graph replay differs from eager"
+    fused_graph = batched.capture(fused=True)
+    batched.out.zero_()
+    fused_graph.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(batched.out, ref_all), "fused graph replay differs from eager"
 
     for _ in range(args.warmup):
-        batched.replay()
+        fused_graph.replay()
+        chain_graph.replay()
         plain.run()
     torch.cuda.synchronize()
 
-    t_graph = timed(batched.replay, args.steps, torch)
+    t_graph = timed(fused_graph.replay, args.steps, torch)
+    t_fused = timed(lambda: batched.run_fused(fused_out), args.steps, torch)
+    t_chain_graph = timed(chain_graph.replay, args.steps, torch)
     t_batched = timed(batched.run, args.steps, torch)
     t_plain = timed(plain.run, args.steps * 2, torch) / 2.0
     ks_graph = args.steps * B / t_graph
+    ks_fused = args.steps * B / t_fused
+    ks_chain_graph = args.steps * B / t_chain_graph
     ks_batched = args.steps * B / t_batched
     ks_plain = args.steps / t_plain
+    fused_gbs = batched.fused_bytes() * args.steps / t_graph / 1e9
 
     # NTT pass kernel: forward digit NTTs of the batch (B*dnum*L limbs of N points);
     # each of the 2 launches reads and writes every limb once.
@@ -152,7 +176,8 @@ def main():
     intt_limb_gbs = limbs_inv * 2 * N * 8 * reps / t_intt / 1e9
 
     result = {
-        "metric": "key-switches/sec (N=2^16, L=24, dnum=3, w=60), batched hipGraph form; NTT limb-GB/s beside",
+        "metric": "key-switches/sec (N=2^16, L=24, dnum=3, w=60), batched fused hipGraph form; plain per-call "
+                  "form and NTT limb-GB/s beside",
         "value": round(ks_graph, 2),
         "unit": "key-switches/s",
         "n_gpus": 1,
@@ -165,11 +190,16 @@ def main():
         "dtype": "int64",
         "data": "synthetic (seeded uniform residues mod 24 primes of 50-60 bits; seeded evaluation keys)",
         "config": {"workload": f"hybrid key-switch N={N} L={L} dnum=3 w=60, Galois g=5 permute", "batch_per_step": B,
-                   "form": "graph-batched", "plain_ks_per_s": round(ks_plain, 2),
-                   "batched_eager_ks_per_s": round(ks_batched, 2),
+                   "form": "fused kernels in a hipGraph, B independent instances per step",
+                   "plain_ks_per_s": round(ks_plain, 2),
+                   "batched_chain_eager_ks_per_s": round(ks_batched, 2),
+                   "batched_chain_graph_ks_per_s": round(ks_chain_graph, 2),
+                   "fused_eager_ks_per_s": round(ks_fused, 2),
+                   "fused_graph_GBps_at_kernel_boundaries": round(fused_gbs, 1),
                    "ntt_limb_GBps": round(ntt_limb_gbs, 1), "intt_limb_GBps": round(intt_limb_gbs, 1),
                    "keyswitch_algorithmic_bytes": batched.algorithmic_bytes() // B,
-                   "keyswitch_chain_bytes": batched.chain_bytes() // B},
+                   "keyswitch_chain_bytes": batched.chain_bytes() // B,
+                   "keyswitch_fused_bytes": batched.fused_bytes() // B},
         "roofline": {"bound": "hbm", "kernel": "k_ntt_pass (forward, 2 launches per NTT)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

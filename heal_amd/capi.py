@@ -77,11 +77,12 @@ _PROTOS = {
     "hx_ntt_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, VP, VP, VP, VP, STREAM],
     "hx_ntt_plan_exec": [ctypes.c_void_p, VP, VP, STREAM],
     "hx_ntt_plan_destroy": [ctypes.c_void_p],
-}
-
-# optional entry points (vendor extensions added over time)
-_OPTIONAL = {
-    "hx_keyswitch_fused": None,
+    "hx_keyswitch_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p, VP, U64, U64,
+                                 STREAM],
+    "hx_keyswitch_key_bytes": [ctypes.c_void_p, I64, ctypes.POINTER(ctypes.c_size_t)],
+    "hx_keyswitch_prepare_key": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
+    "hx_keyswitch_exec": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, VP, STREAM],
+    "hx_keyswitch_plan_destroy": [ctypes.c_void_p],
 }
 
 
@@ -285,6 +286,45 @@ class NttPlan:
     def close(self):
         if self._h:
             check(lib.hx_ntt_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KeySwitchPlan:
+    """Fused hybrid key-switch over a forward and an inverse NttPlan of the same
+    moduli and permutation (hx_keyswitch_* in include/heal_amd.h)."""
+
+    def __init__(self, fwd: NttPlan, inv: NttPlan, perms, dnum: int, base_bits: int):
+        _need_cuda(perms)
+        self._h = ctypes.c_void_p()
+        self._plans = (fwd, inv)  # the fused plan borrows them
+        self.dnum, self.base_bits = int(dnum), int(base_bits)
+        check(lib.hx_keyswitch_plan_create(ctypes.byref(self._h), fwd._h, inv._h, view(perms), self.dnum,
+                                           self.base_bits, stream_of(perms)))
+
+    def prepare_key(self, evk):
+        """evk [dnum, B, 2, m, k] -> device-layout key (an int64 tensor)."""
+        _need_cuda(evk)
+        nbytes = ctypes.c_size_t()
+        check(lib.hx_keyswitch_key_bytes(self._h, int(evk.shape[1]), ctypes.byref(nbytes)))
+        key = torch.empty(nbytes.value // 8, dtype=torch.int64, device=evk.device)
+        check(lib.hx_keyswitch_prepare_key(self._h, view(evk), key.data_ptr(), stream_of(evk)))
+        return key
+
+    def __call__(self, x, key, out):
+        _need_cuda(x, key, out)
+        check(lib.hx_keyswitch_exec(self._h, view(x), key.data_ptr(), key.numel() * key.element_size(), view(out),
+                                    stream_of(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            check(lib.hx_keyswitch_plan_destroy(self._h))
             self._h = ctypes.c_void_p()
 
     def __del__(self):

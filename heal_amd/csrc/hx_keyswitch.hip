@@ -1,0 +1,511 @@
+// hx_keyswitch.hip — fused hybrid key-switch (vendor extension of the HEAL API).
+//
+// Computes, for B independent instances, exactly what the six HEAL calls
+//     g_decomposition -> ntt -> modmul_ttt -> axis_modsum(axis 0) -> intt -> permute
+// compute on the views that chain uses (x [B, N, L] -> digits [B, N, L, D]
+// -> digit NTTs [B, N, D, L] -> products with the evaluation key
+// [D, B, 2, N, L] -> sum over digits [B, 2, N, L] -> INTT -> Galois gather with
+// perms [L, N]), in three kernels:
+//   K1  forward pass over the high index bits, reading x and extracting the
+//       base-2^w digits on load;
+//   K2  forward pass over the low index bits of all D digit columns of a limb
+//       group, the key inner product and digit sum in LDS, and the inverse
+//       pass over the low bits;
+//   K3  inverse pass over the high bits, m^-1 scaling, and the Galois
+//       permutation applied as a per-limb row scatter on store.
+// Between the forward and inverse transforms the values stay in the NTT's
+// network order; the key is permuted into that order once
+// (hx_keyswitch_prepare_key), which is why the forward and inverse plans must
+// share one permutation.  Every elementwise step uses the reference's C++
+// integer semantics, so the result is bit-identical with the six-call chain.
+#include <new>
+
+#include "hx_ntt_kernels.hpp"
+
+struct hx_keyswitch_plan {
+    int device;
+    const hx_ntt_plan* fwd;
+    const hx_ntt_plan* inv;
+    int64_t m, k;
+    int logm, hhi, hlo;
+    int D, w;
+    void* mem;
+    int32_t* galmap;  // [k][m]: output row of coefficient j of limb t
+    mutable void* ws;
+    mutable size_t ws_bytes;
+};
+
+namespace hx {
+namespace {
+
+struct InnerArgs {
+    const uint64_t* src;  // K1 output: rows of D*k*B, column d + D*(t + k*b)
+    uint64_t* dst;        // K3 input:  rows of 2*k*B, column c + 2*(t + k*b)
+    const int64_t* key;   // [D][B][2][k][m] in network order
+    const LimbInfo* limbs;
+    const uint64_t* tw_f;
+    const uint64_t* tw_i;
+    int64_t m, k, B;
+    int32_t logm, hhi, D, G;
+    int64_t n_tg, n_hi;
+};
+
+// (sum + a) % M with C++ truncation, fast for the usual non-negative case
+__device__ __forceinline__ int64_t seq_rem(const ModInfo& mi, int64_t s) {
+    if (s >= 0 && (uint64_t)s < mi.M) return s;
+    if (s >= 0 && mi.M <= (uint64_t(1) << 62) && (uint64_t)s < 2 * mi.M) return s - (int64_t)mi.M;
+    return cpp_rem(mi, s);
+}
+
+template <int H>
+__global__ __launch_bounds__(1024) void k_ks_inner(InnerArgs A) {
+    constexpr int W = 1 << H;
+    constexpr int WP = W + W / 32;
+    constexpr int TL = W / 8;
+    extern __shared__ uint64_t lds[];
+    const int D = A.D, G = A.G;
+    const int NF = G * D;  // forward sub-problems: s = d + D * gi
+    const int NI = 2 * G;  // inverse sub-problems: s = c + 2 * gi
+    uint64_t* xf = lds;
+    uint64_t* xi = lds + NF * WP;
+
+    int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t hi = tile % A.n_hi;
+    tile /= A.n_hi;
+    const int64_t tg = tile % A.n_tg;
+    const int64_t b = tile / A.n_tg;
+    const int64_t t0 = tg * G;
+    const int64_t ncols1 = (int64_t)D * A.k * A.B, ncols2 = 2 * A.k * A.B;
+    const int64_t row0 = hi * W;
+
+    // ---- load the D digit columns of each limb of the group (adjacent in memory)
+    {
+        const int s = threadIdx.x % NF, step = blockDim.x / NF;
+        const int64_t t = t0 + s / D;
+        if (t < A.k) {
+            const uint64_t* src = A.src + row0 * ncols1 + (s % D) + (int64_t)D * (t + A.k * b);
+            for (int w = threadIdx.x / NF; w < W; w += step) xf[s * WP + r8_pad(w)] = src[(int64_t)w * ncols1];
+        }
+    }
+    __syncthreads();
+
+    // ---- forward stages over the low bits (global stages hhi .. logm-1)
+    {
+        const int s = threadIdx.x / TL, lane = threadIdx.x % TL;
+        const bool active = s < NF;
+        const int ss = active ? s : 0;
+        int64_t t = t0 + ss / D;
+        if (t >= A.k) t = A.k - 1;  // a partial last group computes on stale values, never stored
+        const LimbInfo& L = A.limbs[t];
+        TwRow R;
+        R.tw = (const ulonglong2*)(A.tw_f + 2 * t * A.m);
+        R.top = hi;
+        R.logm = A.logm;
+        R.g0 = A.hhi;
+        r8_all_rounds<H, false>(xf + ss * WP, lane, R, L.q, limb_mode(L), active);
+    }
+
+    // ---- key inner product and sum over digits, in the chain's integer semantics:
+    //      prod_d = (T_DP(D_d) * T_DP(key_d)) % q      (modmul_ttt)
+    //      sum    = (sum + prod_d) % q, d = 0..D-1     (axis_modsum)
+    //      INTT input = residue of sum                  (intt load)
+    for (int e = threadIdx.x; e < G * W; e += blockDim.x) {
+        const int gi = e / W, w = e % W;
+        const int64_t t = t0 + gi;
+        if (t >= A.k) continue;
+        const LimbInfo& L = A.limbs[t];
+        const ModInfo mi = limb_modinfo(L);
+        const bool lazy = limb_mode(L) == MODE_L64;
+        const int64_t j = row0 + w;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            int64_t sum = 0;
+            for (int d = 0; d < D; ++d) {
+                const int64_t kv = A.key[((((int64_t)d * A.B + b) * 2 + c) * A.k + t) * A.m + j];
+                const uint64_t v = xf[(d + D * gi) * WP + r8_pad(w)];
+                const uint64_t x = lazy ? lazy_reduce(v, L.q) : v;
+                int64_t prod;
+                if (kv >= 0 && (uint64_t)kv < L.q) prod = (int64_t)mulmod(mi, x, (uint64_t)kv);
+                else prod = cpp_mulmod(mi, (int64_t)x, kv);
+                sum = seq_rem(mi, (int64_t)((uint64_t)sum + (uint64_t)prod));
+            }
+            uint64_t v = (uint64_t)sum;
+            if (sum < 0) {
+                uint64_t r = reduce64(mi, uabs64(sum));
+                v = r ? L.q - r : 0;
+            }
+            xi[(c + 2 * gi) * WP + r8_pad(w)] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- inverse stages over the low bits (global stages 0 .. hlo-1)
+    {
+        const int s = threadIdx.x / TL, lane = threadIdx.x % TL;
+        const bool active = s < NI;
+        const int ss = active ? s : 0;
+        int64_t t = t0 + ss / 2;
+        if (t >= A.k) t = A.k - 1;
+        const LimbInfo& L = A.limbs[t];
+        TwRow R;
+        R.tw = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
+        R.top = hi;
+        R.logm = A.logm;
+        R.g0 = 0;
+        r8_all_rounds<H, true>(xi + ss * WP, lane, R, L.q, limb_mode(L), active);
+    }
+
+    // ---- store (residues stay lazy for the next radix-8 pass)
+    {
+        const int s = threadIdx.x % NI, step = blockDim.x / NI;
+        const int64_t t = t0 + s / 2;
+        if (t >= A.k) return;
+        uint64_t* dst = A.dst + row0 * ncols2 + (s % 2) + 2 * (t + A.k * b);
+        for (int w = threadIdx.x / NI; w < W; w += step) dst[(int64_t)w * ncols2] = xi[s * WP + r8_pad(w)];
+    }
+}
+
+// key[((d*B + b)*2 + c)*k + t][j] = evk[d, b, c, rowmap[j], t]
+struct KeyArgs {
+    const int64_t* evk;
+    int64_t s[5];  // strides of the [D, B, 2, m, k] view
+    int64_t* key;
+    const int32_t* rowmap;  // null: identity
+    int64_t D, B, m, k;
+};
+
+__global__ void k_ks_prepare_key(KeyArgs K) {
+    const int64_t n = K.D * K.B * 2 * K.k * K.m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = e % K.m, r = e / K.m;
+        int64_t t = r % K.k;
+        r /= K.k;
+        int64_t c = r % 2;
+        r /= 2;
+        int64_t b = r % K.B, d = r / K.B;
+        int64_t u = K.rowmap ? (int64_t)K.rowmap[j] : j;
+        K.key[e] = K.evk[d * K.s[0] + b * K.s[1] + c * K.s[2] + u * K.s[3] + t * K.s[4]];
+    }
+}
+
+// Galois maps: counts per (limb, row), then the inverse map galmap[t][perms[t, n]] = n.
+__global__ void k_ks_gal_count(PlanSrc perms, int64_t k, int64_t m, int32_t* counts, unsigned long long* flags) {
+    const int64_t n = k * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = e / m, u = e - t * m;
+        int64_t v = ld_elem(perms.ptr, t * perms.s0 + u * perms.s1, perms.elem32);
+        if (v < 0 || v >= m) {
+            atomicAdd(&flags[0], 1ull);
+            continue;
+        }
+        atomicAdd(&counts[t * m + v], 1);
+    }
+}
+
+__global__ void k_ks_gal_map(PlanSrc perms, int64_t k, int64_t m, const int32_t* counts, int32_t* galmap,
+                             unsigned long long* flags) {
+    const int64_t n = k * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        if (counts[e] != 1) atomicAdd(&flags[1], 1ull);
+        int64_t t = e / m, u = e - t * m;
+        int64_t v = ld_elem(perms.ptr, t * perms.s0 + u * perms.s1, perms.elem32);
+        if (v >= 0 && v < m) galmap[t * m + v] = (int32_t)u;
+    }
+}
+
+// forward and inverse plans must agree on moduli and permutation
+__global__ void k_ks_compare(const LimbInfo* a, const LimbInfo* b, int64_t k, const int32_t* ma, const int32_t* mb,
+                             int64_t m, unsigned long long* flags) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k + m; e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < k) {
+            if (a[e].q != b[e].q) atomicAdd(&flags[2], 1ull);
+        } else if (ma && mb && ma[e - k] != mb[e - k]) {
+            atomicAdd(&flags[2], 1ull);
+        }
+    }
+}
+
+}  // namespace
+
+hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_plan* inv, const hx_view* perms,
+                                         uint64_t dnum, uint64_t base_bits, hipStream_t st) {
+    if (!fwd || !inv) raise(HX_ERR_INVALID_ARGUMENT, "plan pointers must not be null.");
+    if (fwd->inverse || !inv->inverse) raise(HX_ERR_INVALID_ARGUMENT, "need a forward and an inverse NTT plan");
+    if (fwd->dtype != HX_INT64 || inv->dtype != HX_INT64)
+        raise(HX_ERR_INVALID_ARGUMENT, "the fused key-switch works on int64 tensors");
+    if (fwd->m != inv->m || fwd->k != inv->k) raise(HX_ERR_INVALID_ARGUMENT, "plans differ in m or k");
+    if (fwd->device != inv->device) raise(HX_ERR_INVALID_ARGUMENT, "plans belong to different devices");
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != fwd->device) raise(HX_ERR_INVALID_ARGUMENT, "plans belong to another device");
+    const int64_t m = fwd->m, k = fwd->k;
+    const int logm = fwd->logm;
+    if (logm < 12 || logm > 16) raise(HX_ERR_INVALID_ARGUMENT, "the fused key-switch needs 2^12 <= m <= 2^16");
+    const bool fmap = fwd->perm_kind == PERM_MAP, imap = inv->perm_kind == PERM_MAP;
+    if (!((fwd->perm_kind == PERM_IDENTITY || fmap) && fwd->perm_kind == inv->perm_kind))
+        raise(HX_ERR_INVALID_ARGUMENT, "forward and inverse plans must share one permutation");
+    if (dnum < 1 || dnum > 64) raise(HX_ERR_INVALID_ARGUMENT, "dnum must be in [1, 64]");
+    if (base_bits >= 64) raise(HX_ERR_INVALID_ARGUMENT, "base_bits must be < 64");
+    check_view(perms, "perms");
+    if (perms->dtype != HX_INT32 && perms->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, "perms must be integer");
+    if (perms->ndim != 2 || perms->dims[0] != k || perms->dims[1] != m)
+        raise(HX_ERR_INVALID_ARGUMENT, "Perms must have shape [l, m] where l and m match a.shape at elementwise and perm axes.");
+
+    hx_keyswitch_plan* plan = new hx_keyswitch_plan;
+    std::memset(plan, 0, sizeof *plan);
+    plan->device = dev;
+    plan->fwd = fwd;
+    plan->inv = inv;
+    plan->m = m;
+    plan->k = k;
+    plan->logm = logm;
+    plan->hlo = logm / 2;
+    plan->hhi = logm - plan->hlo;
+    plan->D = (int)dnum;
+    plan->w = (int)base_bits;
+    const size_t map_bytes = ((size_t)k * m * 4 + 255) / 256 * 256;
+    check_hip(hipMalloc(&plan->mem, 2 * map_bytes + 256), "hipMalloc(keyswitch plan)");
+    plan->galmap = (int32_t*)plan->mem;
+    int32_t* counts = (int32_t*)((char*)plan->mem + map_bytes);
+    unsigned long long* flags = (unsigned long long*)((char*)plan->mem + 2 * map_bytes);
+    try {
+        check_hip(hipMemsetAsync(counts, 0, map_bytes + 256, st), "hipMemsetAsync");
+        PlanSrc ps = plan_src(perms);
+        unsigned g = grid_for(k * m, 256, 8192);
+        hipLaunchKernelGGL(k_ks_gal_count, dim3(g), dim3(256), 0, st, ps, k, m, counts, flags);
+        hipLaunchKernelGGL(k_ks_gal_map, dim3(g), dim3(256), 0, st, ps, k, m, counts, plan->galmap, flags);
+        hipLaunchKernelGGL(k_ks_compare, dim3(grid_for(k + m, 256, 1024)), dim3(256), 0, st, fwd->limbs, inv->limbs, k,
+                           fmap ? fwd->rowmap : nullptr, imap ? inv->rowmap : nullptr, m, flags);
+        check_hip(hipGetLastError(), "keyswitch plan launch");
+        unsigned long long h[3];
+        check_hip(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (h[0]) raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
+        if (h[1]) raise(HX_ERR_INVALID_ARGUMENT, "the fused key-switch needs every row of perms to be a permutation");
+        if (h[2]) raise(HX_ERR_INVALID_ARGUMENT, "forward and inverse plans differ in moduli or permutation");
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(plan->mem);
+        delete plan;
+        throw;
+    }
+    return plan;
+}
+
+void keyswitch_plan_destroy(hx_keyswitch_plan* plan) {
+    if (!plan) return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(plan->mem);
+    if (plan->ws) {
+        (void)hipFreeAsync(plan->ws, nullptr);
+        (void)hipDeviceSynchronize();
+    }
+    delete plan;
+}
+
+static void require_i64(const hx_view* v, const char* name, int ndim) {
+    check_view(v, name);
+    if (v->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must be int64");
+    if (v->ndim != ndim) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " has the wrong number of dims");
+}
+
+size_t keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    if (batch < 1) raise(HX_ERR_INVALID_ARGUMENT, "batch must be positive");
+    return (size_t)plan->D * (size_t)batch * 2 * (size_t)plan->k * (size_t)plan->m * 8;
+}
+
+void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hipStream_t st) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    require_i64(evk, "evk", 5);
+    if (!key) raise(HX_ERR_INVALID_ARGUMENT, "key pointer must not be null.");
+    if (evk->dims[0] != plan->D || evk->dims[2] != 2 || evk->dims[3] != plan->m || evk->dims[4] != plan->k)
+        raise(HX_ERR_INVALID_ARGUMENT, "evk must have shape [dnum, B, 2, m, k]");
+    KeyArgs K;
+    K.evk = (const int64_t*)evk->data;
+    for (int i = 0; i < 5; ++i) K.s[i] = evk->strides[i];
+    K.key = (int64_t*)key;
+    K.rowmap = plan->fwd->perm_kind == PERM_MAP ? plan->fwd->rowmap : nullptr;
+    K.D = plan->D;
+    K.B = evk->dims[1];
+    K.m = plan->m;
+    K.k = plan->k;
+    hipLaunchKernelGGL(k_ks_prepare_key, dim3(grid_for(K.D * K.B * 2 * K.k * K.m, 256, 8192)), dim3(256), 0, st, K);
+    check_hip(hipGetLastError(), "prepare key launch");
+}
+
+void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                    const hx_view* out, hipStream_t st) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    require_i64(x, "x", 3);
+    require_i64(out, "out", 4);
+    if (!key) raise(HX_ERR_INVALID_ARGUMENT, "key pointer must not be null.");
+    const int64_t B = x->dims[0], m = plan->m, k = plan->k;
+    const int D = plan->D;
+    if (B > 0 && key_bytes < keyswitch_key_bytes(plan, B))
+        raise(HX_ERR_INVALID_ARGUMENT, "key buffer smaller than a key for this batch");
+    if (x->dims[1] != m || x->dims[2] != k) raise(HX_ERR_INVALID_ARGUMENT, "x must have shape [B, m, k]");
+    if (out->dims[0] != B || out->dims[1] != 2 || out->dims[2] != m || out->dims[3] != k)
+        raise(HX_ERR_INVALID_ARGUMENT, "out must have shape [B, 2, m, k]");
+    if (views_overlap(x, out)) raise(HX_ERR_INVALID_ARGUMENT, "out must not overlap x");
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != plan->device) raise(HX_ERR_INVALID_ARGUMENT, "plan belongs to another device");
+    if (B == 0) return;
+    const int64_t ncols1 = (int64_t)D * k * B, ncols2 = 2 * k * B;
+    const size_t bytes = (size_t)m * (size_t)(ncols1 + ncols2) * 8;
+    if (plan->ws_bytes < bytes) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone)
+            raise(HX_ERR_RUNTIME, "key-switch workspace must be sized by one execution before graph capture");
+        if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync(ks workspace)");
+        plan->ws = nullptr;
+        plan->ws_bytes = 0;
+        check_hip(hipMallocAsync(&plan->ws, bytes, st), "hipMallocAsync(ks workspace)");
+        plan->ws_bytes = bytes;
+    }
+    uint64_t* tmp1 = (uint64_t*)plan->ws;
+    uint64_t* tmp2 = tmp1 + (size_t)m * ncols1;
+    const hx_ntt_plan* F = plan->fwd;
+    const hx_ntt_plan* I = plan->inv;
+
+    // K1: digits of x, forward pass over the high bits
+    PassArgs P1;
+    std::memset(&P1, 0, sizeof P1);
+    P1.src.ptr = x->data;
+    P1.src.row_stride = x->strides[1];
+    P1.src.cstride[0] = 0;              // digit
+    P1.src.cstride[1] = x->strides[2];  // limb
+    P1.src.cstride[2] = x->strides[0];  // instance
+    P1.src.kind = 3;
+    P1.dst = tmp_addr(tmp1, ncols1);
+    P1.ncols = ncols1;
+    P1.cdim[0] = D;
+    P1.cdim[1] = k;
+    P1.cdim[2] = B;
+    P1.limb_axis = 1;
+    P1.gd_axis = 0;
+    P1.gd_w = plan->w;
+    P1.gd_D = D;
+    P1.gd_status = &device_status()->gdecomp_overflow;
+    P1.inverse = 0;
+    P1.logm = plan->logm;
+    P1.H = plan->hhi;
+    P1.b0 = plan->hlo;
+    P1.stage0 = 0;
+    P1.bot_count = int64_t(1) << plan->hlo;
+    P1.reduce_in = 1;
+    P1.lazy_out = 1;
+    P1.limbs = F->limbs;
+    P1.tw = F->tw;
+    P1.m = m;
+    P1.S_cols = tile_subproblems(P1.H, 2, ncols1 >= 16);
+    P1.S_bot = 1;
+    launch_pass(P1, st);
+
+    // K2: forward low bits, key inner product, inverse low bits
+    InnerArgs A;
+    A.src = tmp1;
+    A.dst = tmp2;
+    A.key = (const int64_t*)key;
+    A.limbs = F->limbs;
+    A.tw_f = F->tw;
+    A.tw_i = I->tw;
+    A.m = m;
+    A.k = k;
+    A.B = B;
+    A.logm = plan->logm;
+    A.hhi = plan->hhi;
+    A.D = D;
+    // limb group: about 12 forward sub-problems per workgroup
+    int G = std::max(1, std::min(8, 12 / D));
+    if (G > k) G = (int)k;
+    A.G = G;
+    A.n_tg = (k + G - 1) / G;
+    A.n_hi = int64_t(1) << plan->hhi;
+    const int H = plan->hlo;
+    const int TL = (1 << H) / 8;
+    const int threads = std::max(G * D, 2 * G) * TL;
+    const size_t lds = (size_t)(G * D + 2 * G) * ((1 << H) + (1 << H) / 32) * 8;
+    if (threads > 1024 || lds > 64 * 1024)
+        raise(HX_ERR_INVALID_ARGUMENT, "dnum too large for the fused key-switch tile");
+    const int64_t blocks = A.n_hi * A.n_tg * B;
+    if (H == 8) hipLaunchKernelGGL(k_ks_inner<8>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
+    else if (H == 7) hipLaunchKernelGGL(k_ks_inner<7>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
+    else hipLaunchKernelGGL(k_ks_inner<6>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
+    check_hip(hipGetLastError(), "keyswitch inner launch");
+
+    // K3: inverse pass over the high bits, m^-1, Galois scatter into out
+    PassArgs P3;
+    std::memset(&P3, 0, sizeof P3);
+    P3.src = tmp_addr(tmp2, ncols2);
+    P3.dst.ptr = out->data;
+    P3.dst.row_stride = out->strides[2];
+    P3.dst.cstride[0] = out->strides[1];  // output polynomial c
+    P3.dst.cstride[1] = out->strides[3];  // limb
+    P3.dst.cstride[2] = out->strides[0];  // instance
+    P3.dst.kind = 0;
+    P3.ncols = ncols2;
+    P3.cdim[0] = 2;
+    P3.cdim[1] = k;
+    P3.cdim[2] = B;
+    P3.limb_axis = 1;
+    P3.inverse = 1;
+    P3.logm = plan->logm;
+    P3.H = plan->hhi;
+    P3.b0 = plan->hlo;
+    P3.stage0 = plan->hlo;
+    P3.bot_count = int64_t(1) << plan->hlo;
+    P3.scale_out = 1;
+    P3.dst_map_limb = plan->galmap;
+    P3.limbs = I->limbs;
+    P3.tw = I->tw;
+    P3.m = m;
+    P3.S_cols = tile_subproblems(P3.H, 2, ncols2 >= 16);
+    P3.S_bot = 1;
+    launch_pass(P3, st);
+}
+
+}  // namespace hx
+
+using namespace hx;
+
+extern "C" {
+
+int hx_keyswitch_plan_create(hx_keyswitch_plan** out, const hx_ntt_plan* fwd, const hx_ntt_plan* inv,
+                             const hx_view* perms, uint64_t dnum, uint64_t base_bits, hx_stream stream) {
+    HX_API_BEGIN
+    if (!out) raise(HX_ERR_INVALID_ARGUMENT, "out pointer must not be null.");
+    *out = nullptr;
+    *out = keyswitch_plan_create(fwd, inv, perms, dnum, base_bits, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch, size_t* bytes) {
+    HX_API_BEGIN
+    if (!bytes) raise(HX_ERR_INVALID_ARGUMENT, "bytes pointer must not be null.");
+    *bytes = keyswitch_key_bytes(plan, batch);
+    HX_API_END
+}
+
+int hx_keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hx_stream stream) {
+    HX_API_BEGIN
+    keyswitch_prepare_key(plan, evk, key, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                      const hx_view* out, hx_stream stream) {
+    HX_API_BEGIN
+    keyswitch_exec(plan, x, key, key_bytes, out, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_plan_destroy(hx_keyswitch_plan* plan) {
+    HX_API_BEGIN
+    keyswitch_plan_destroy(plan);
+    HX_API_END
+}
+
+}  // extern "C"

@@ -22,386 +22,13 @@
 //
 // A plan (hx_ntt_plan) snapshots p, twiddles, perm and m_inv on the device, so
 // executing it reads only the data columns and the packed twiddle table.
-#include <cstring>
 #include <new>
 #include <vector>
 
-#include "hx_internal.hpp"
-
-struct LimbInfo {
-    uint64_t q;
-    uint64_t d;      // Möller–Granlund normalised divisor
-    uint64_t v;      // and its reciprocal
-    uint32_t s;
-    uint32_t small;  // 1: q < 2^31, butterflies in 32-bit arithmetic
-    uint64_t minv;   // m^-1 mod q (inverse plans)
-    uint64_t minv_p; // its Shoup companion (32- or 64-bit)
-    uint64_t pad[2];
-};
-
-struct hx_ntt_plan {
-    int device;
-    int inverse;
-    int dtype;
-    int logm;
-    int64_t m;
-    int64_t k;
-    int perm_kind;        // PERM_* below
-    void* mem;            // single device allocation
-    // Intermediate buffer between passes.  A plan executes on one stream at a
-    // time (like an FFT plan); the workspace grows on first use.
-    mutable void* ws;
-    mutable size_t ws_bytes;
-    LimbInfo* limbs;      // [k]
-    uint64_t* tw;         // [k][m][2]: (w mod q, companion)
-    int32_t* rowmap;      // [m] (see perm_kind)
-};
+#include "hx_ntt_kernels.hpp"
 
 namespace hx {
-namespace {
 
-enum : int {
-    PERM_IDENTITY = 0,
-    PERM_MAP = 1,          // a permutation: rowmap[perm[u]] = u (fused)
-    PERM_GATHER_FWD = 2,   // forward, not a permutation: out[u] = x[rowmap[u]] (rowmap = perm)
-    PERM_SCATTER_INV = 3,  // inverse, not a permutation: x[j] = a[rowmap[j]] or old result if -1
-};
-
-constexpr int kThreads = 256;
-
-__device__ __forceinline__ ModInfo limb_modinfo(const LimbInfo& L) {
-    ModInfo mi;
-    mi.M = L.q;
-    mi.d = L.d;
-    mi.v = L.v;
-    mi.s = L.s;
-    mi.pad = 0;
-    return mi;
-}
-
-template <typename T>
-__device__ __forceinline__ int64_t ldv(const void* base, int64_t off) {
-    return (int64_t)((const T*)base)[off];
-}
-
-__device__ __forceinline__ int64_t ld_elem(const void* base, int64_t off, int elem32) {
-    return elem32 ? (int64_t)((const int32_t*)base)[off] : ((const int64_t*)base)[off];
-}
-
-// Exact residue of a signed value (inputs of the NTT are non-negative; a
-// negative one is reduced mathematically).
-__device__ __forceinline__ uint64_t residue(const ModInfo& mi, int64_t x) {
-    if (x >= 0) return reduce64(mi, (uint64_t)x);
-    uint64_t r = reduce64(mi, uabs64(x));
-    return r ? mi.M - r : 0;
-}
-
-// ------------------------------------------------------------ plan kernels
-struct PlanSrc {
-    const void* ptr;
-    int64_t s0, s1;
-    int32_t elem32;
-};
-
-__global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, LimbInfo* limbs,
-                             unsigned long long* flags) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= k) return;
-    int64_t q = ld_elem(p.ptr, t * p.s0, p.elem32);
-    LimbInfo L;
-    memset(&L, 0, sizeof L);
-    if (q <= 1) {
-        atomicAdd(&flags[0], 1ull);
-        q = 2;  // keeps the table well-formed; the plan is rejected on the host
-    }
-    ModInfo mi = make_modinfo((uint64_t)q);
-    L.q = (uint64_t)q;
-    L.d = mi.d;
-    L.v = mi.v;
-    L.s = mi.s;
-    L.small = q < (int64_t(1) << 31);
-    if (has_minv) {
-        uint64_t mv = residue(mi, ld_elem(minv.ptr, t * minv.s0, minv.elem32));
-        L.minv = mv;
-        L.minv_p = L.small ? divq128(mi, 0, mv << 32) : shoup_companion(mi, mv);
-    }
-    limbs[t] = L;
-}
-
-__global__ void k_plan_twiddles(PlanSrc tw, int64_t k, int64_t m, const LimbInfo* limbs, uint64_t* out) {
-    int64_t n = k * m;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        int64_t t = e / m, i = e - t * m;
-        LimbInfo L = limbs[t];
-        ModInfo mi = limb_modinfo(L);
-        uint64_t w = residue(mi, ld_elem(tw.ptr, t * tw.s0 + i * tw.s1, tw.elem32));
-        uint64_t wp = L.small ? divq128(mi, 0, w << 32) : shoup_companion(mi, w);
-        out[2 * e] = w;
-        out[2 * e + 1] = wp;
-    }
-}
-
-__global__ void k_plan_perm_count(PlanSrc perm, int64_t m, int32_t* counts, unsigned long long* flags) {
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < m; u += (int64_t)gridDim.x * blockDim.x) {
-        int64_t v = ld_elem(perm.ptr, u * perm.s0, perm.elem32);
-        if (v < 0 || v >= m) {
-            atomicAdd(&flags[1], 1ull);
-            continue;
-        }
-        if (v != u) atomicAdd(&flags[2], 1ull);
-        atomicAdd(&counts[v], 1);
-    }
-}
-
-__global__ void k_plan_perm_check(const int32_t* counts, int64_t m, unsigned long long* flags) {
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
-        if (counts[j] != 1) atomicAdd(&flags[3], 1ull);
-}
-
-// mode 0: map[perm[u]] = u; mode 1: map[u] = perm[u]; mode 2: map[perm[u]] = max u (map pre-filled with -1)
-__global__ void k_plan_map(PlanSrc perm, int64_t m, int32_t* map, int mode) {
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < m; u += (int64_t)gridDim.x * blockDim.x) {
-        int32_t v = (int32_t)ld_elem(perm.ptr, u * perm.s0, perm.elem32);
-        if (mode == 0) map[v] = (int32_t)u;
-        else if (mode == 1) map[u] = v;
-        else atomicMax(&map[v], (int32_t)u);
-    }
-}
-
-__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t val) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = val;
-}
-
-// ------------------------------------------------------------ pass kernel
-// Column addressing: a column is (i, j, t); the three axes are listed in
-// `order` (innermost first) with their sizes and strides for src and dst.
-// The intermediate buffer uses rows of `ncols` uint64 with the column index
-// innermost.
-struct ColAddr {
-    const void* ptr;
-    int64_t row_stride;
-    int64_t cstride[3];
-    int32_t kind;  // 0: int64 tensor, 1: int32 tensor, 2: uint64 intermediate
-};
-
-struct PassArgs {
-    ColAddr src;
-    ColAddr dst;
-    ColAddr alt;             // old result, read where a scatter leaves a row untouched
-    int64_t ncols;
-    int64_t cdim[3];
-    int32_t limb_axis;       // which of the 3 column axes is t
-    int32_t inverse;
-    int32_t logm;
-    int32_t b0;              // lowest index bit of the window
-    int32_t H;               // window bits
-    int32_t stage0;          // first global stage of this pass (h for forward, g for inverse)
-    int32_t S_cols, S_bot;   // tile = S_cols columns x S_bot "bot" values
-    int64_t bot_count;       // 2^b0
-    int64_t n_bot_tiles, n_col_tiles;
-    int32_t reduce_in;
-    int32_t scale_out;
-    const int32_t* src_map;  // row gather on load (or null)
-    const int32_t* dst_map;  // row scatter on store (or null)
-    int32_t src_map_alt;     // src_map entries of -1 read `alt` at the same row
-    const LimbInfo* limbs;
-    const uint64_t* tw;
-    int64_t m;
-};
-
-__device__ __forceinline__ int64_t col_offset(const ColAddr& c, const int64_t* coord) {
-    return coord[0] * c.cstride[0] + coord[1] * c.cstride[1] + coord[2] * c.cstride[2];
-}
-
-__device__ __forceinline__ int64_t load_any(const ColAddr& c, int64_t off) {
-    if (c.kind == 1) return ((const int32_t*)c.ptr)[off];
-    return ((const int64_t*)c.ptr)[off];
-}
-
-__device__ __forceinline__ void store_any(const ColAddr& c, int64_t off, uint64_t v) {
-    if (c.kind == 1) ((int32_t*)c.ptr)[off] = (int32_t)v;
-    else ((uint64_t*)c.ptr)[off] = v;
-}
-
-__global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs P) {
-    extern __shared__ uint64_t xs[];  // [S][W + 1]
-    __shared__ int64_t s_src_off[64], s_dst_off[64], s_alt_off[64], s_jbase[64];
-    __shared__ int32_t s_limb[64], s_valid[64];
-
-    const int W = 1 << P.H;
-    const int WP = W + 1;
-    const int S = P.S_cols * P.S_bot;
-
-    int64_t tile = blockIdx.x;
-    const int64_t col_tile = tile % P.n_col_tiles;
-    tile /= P.n_col_tiles;
-    const int64_t bot_tile = tile % P.n_bot_tiles;
-    const int64_t top = tile / P.n_bot_tiles;
-
-    if (threadIdx.x < S) {
-        int s = threadIdx.x;
-        int cs = s % P.S_cols, bs = s / P.S_cols;
-        int64_t col = col_tile * P.S_cols + cs;
-        int64_t bot = bot_tile * P.S_bot + bs;
-        int valid = col < P.ncols && bot < P.bot_count;
-        int64_t c = valid ? col : 0;
-        int64_t coord[3];
-        coord[0] = c % P.cdim[0];
-        c /= P.cdim[0];
-        coord[1] = c % P.cdim[1];
-        coord[2] = c / P.cdim[1];
-        int64_t lin = valid ? col : 0;
-        // the intermediate buffer addresses columns by their linear index
-        int64_t lcoord[3] = {lin, 0, 0};
-        s_src_off[s] = P.src.kind == 2 ? col_offset(P.src, lcoord) : col_offset(P.src, coord);
-        s_dst_off[s] = P.dst.kind == 2 ? col_offset(P.dst, lcoord) : col_offset(P.dst, coord);
-        s_alt_off[s] = P.alt.ptr ? col_offset(P.alt, coord) : 0;
-        s_limb[s] = (int32_t)coord[P.limb_axis];
-        s_valid[s] = valid;
-        s_jbase[s] = (top << (P.b0 + P.H)) + (valid ? bot : 0);
-    }
-    __syncthreads();
-
-    // ---- load: consecutive threads take consecutive sub-problems of one row
-    for (int e = threadIdx.x; e < S * W; e += blockDim.x) {
-        int s = e % S, w = e / S;
-        if (!s_valid[s]) continue;
-        int64_t j = s_jbase[s] + ((int64_t)w << P.b0);
-        int64_t x;
-        if (P.src_map) {
-            int32_t r = P.src_map[j];
-            if (r >= 0) x = load_any(P.src, s_src_off[s] + (int64_t)r * P.src.row_stride);
-            else x = load_any(P.alt, s_alt_off[s] + j * P.alt.row_stride);
-        } else {
-            x = load_any(P.src, s_src_off[s] + j * P.src.row_stride);
-        }
-        uint64_t v = (uint64_t)x;
-        if (P.reduce_in) {
-            LimbInfo L = P.limbs[s_limb[s]];
-            if (x < 0 || (uint64_t)x >= L.q) v = residue(limb_modinfo(L), x);
-        }
-        xs[s * WP + w] = v;
-    }
-    __syncthreads();
-
-    // ---- H radix-2 stages in LDS; a wave stays inside one sub-problem
-    const int half_count = W >> 1;
-    for (int st = 0; st < P.H; ++st) {
-        // forward: window bit H-1-st; inverse: window bit st
-        const int bit = P.inverse ? st : (P.H - 1 - st);
-        const int half = 1 << bit;
-        const int g = P.stage0 + st;  // global stage
-        for (int bf = threadIdx.x; bf < S * half_count; bf += blockDim.x) {
-            int s = bf / half_count, i = bf - s * half_count;
-            if (!s_valid[s]) continue;
-            int lo = ((i >> bit) << (bit + 1)) | (i & (half - 1));
-            int hi = lo + half;
-            int64_t jlo = s_jbase[s] + ((int64_t)lo << P.b0);
-            int64_t tw_idx = P.inverse ? ((int64_t(1) << (P.logm - 1 - g)) + (jlo >> (g + 1)))
-                                       : ((int64_t(1) << g) + (jlo >> (P.logm - g)));
-            const int limb = s_limb[s];
-            const uint64_t* twp = P.tw + 2 * ((int64_t)limb * P.m + tw_idx);
-            uint64_t w = twp[0], wp = twp[1];
-            const LimbInfo& L = P.limbs[limb];
-            uint64_t q = L.q;
-            uint64_t u = xs[s * WP + lo], v = xs[s * WP + hi];
-            uint64_t nu, nv;
-            if (L.small) {
-                uint32_t q32 = (uint32_t)q;
-                if (!P.inverse) {
-                    uint32_t t = shoup_mul32((uint32_t)v, (uint32_t)w, (uint32_t)wp, q32);
-                    uint32_t a = (uint32_t)u + t;
-                    nu = a >= q32 ? a - q32 : a;
-                    nv = (uint32_t)u >= t ? (uint32_t)u - t : (uint32_t)u + q32 - t;
-                } else {
-                    uint32_t a = (uint32_t)u + (uint32_t)v;
-                    nu = a >= q32 ? a - q32 : a;
-                    uint32_t d = (uint32_t)u >= (uint32_t)v ? (uint32_t)u - (uint32_t)v : (uint32_t)u + q32 - (uint32_t)v;
-                    nv = shoup_mul32(d, (uint32_t)w, (uint32_t)wp, q32);
-                }
-            } else {
-                if (!P.inverse) {
-                    uint64_t t = shoup_mul(v, w, wp, q);
-                    nu = addmod(u, t, q);
-                    nv = submod(u, t, q);
-                } else {
-                    nu = addmod(u, v, q);
-                    nv = shoup_mul(submod(u, v, q), w, wp, q);
-                }
-            }
-            xs[s * WP + lo] = nu;
-            xs[s * WP + hi] = nv;
-        }
-        __syncthreads();
-    }
-
-    // ---- store
-    for (int e = threadIdx.x; e < S * W; e += blockDim.x) {
-        int s = e % S, w = e / S;
-        if (!s_valid[s]) continue;
-        int64_t j = s_jbase[s] + ((int64_t)w << P.b0);
-        int64_t row = P.dst_map ? (int64_t)P.dst_map[j] : j;
-        uint64_t v = xs[s * WP + w];
-        if (P.scale_out) {
-            const LimbInfo& L = P.limbs[s_limb[s]];
-            if (L.small) v = shoup_mul32((uint32_t)v, (uint32_t)L.minv, (uint32_t)L.minv_p, (uint32_t)L.q);
-            else v = shoup_mul(v, L.minv, L.minv_p, L.q);
-        }
-        store_any(P.dst, s_dst_off[s] + row * P.dst.row_stride, v);
-    }
-}
-
-// dst[u][col] = src[map[u]][col] (forward gather for non-permutation perms)
-struct GatherArgs {
-    ColAddr src;
-    ColAddr dst;
-    int64_t ncols;
-    int64_t cdim[3];
-    const int32_t* map;
-    int64_t m;
-};
-
-__global__ void k_row_gather(GatherArgs G) {
-    int64_t n = G.m * G.ncols;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        int64_t u = e / G.ncols, col = e - u * G.ncols;
-        int64_t c = col, coord[3];
-        coord[0] = c % G.cdim[0];
-        c /= G.cdim[0];
-        coord[1] = c % G.cdim[1];
-        coord[2] = c / G.cdim[1];
-        int64_t lc[3] = {col, 0, 0};
-        int64_t so = G.src.kind == 2 ? col_offset(G.src, lc) : col_offset(G.src, coord);
-        int64_t dof = G.dst.kind == 2 ? col_offset(G.dst, lc) : col_offset(G.dst, coord);
-        int64_t v = load_any(G.src, so + (int64_t)G.map[u] * G.src.row_stride);
-        store_any(G.dst, dof + u * G.dst.row_stride, (uint64_t)v);
-    }
-}
-
-// ------------------------------------------------------------ host side
-int ilog2_exact(int64_t m) {
-    int l = 0;
-    while ((int64_t(1) << l) < m) ++l;
-    return ((int64_t(1) << l) == m) ? l : -1;
-}
-
-void require_int_view(const hx_view* v, const char* name) {
-    check_view(v, name);
-    if (v->dtype != HX_INT32 && v->dtype != HX_INT64)
-        raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + ": NTT needs int32 or int64 tensors");
-}
-
-PlanSrc plan_src(const hx_view* v) {
-    PlanSrc s;
-    s.ptr = v->data;
-    s.s0 = v->ndim > 0 ? v->strides[0] : 0;
-    s.s1 = v->ndim > 1 ? v->strides[1] : 0;
-    s.elem32 = v->dtype == HX_INT32;
-    return s;
-}
-
-}  // namespace
 
 // validate_ntt_inputs (example_impl/ntt_impl.cpp:14-43), for the plan's inputs
 static void validate_plan_inputs(int inverse, const hx_view* p, const hx_view* perm, const hx_view* tw,
@@ -515,43 +142,7 @@ void ntt_plan_destroy(hx_ntt_plan* plan) {
     delete plan;
 }
 
-namespace {
 
-ColAddr tensor_addr(const hx_view* v, const int* order) {
-    ColAddr c;
-    c.ptr = v->data;
-    c.row_stride = v->strides[1];
-    const int axes[3] = {0, 2, 3};
-    for (int i = 0; i < 3; ++i) c.cstride[i] = v->strides[axes[order[i]]];
-    c.kind = v->dtype == HX_INT32 ? 1 : 0;
-    return c;
-}
-
-ColAddr tmp_addr(void* ptr, int64_t ncols) {
-    ColAddr c;
-    c.ptr = ptr;
-    c.row_stride = ncols;
-    c.cstride[0] = 1;
-    c.cstride[1] = 0;
-    c.cstride[2] = 0;
-    c.kind = 2;
-    return c;
-}
-
-void launch_pass(PassArgs& P, hipStream_t st) {
-    const int W = 1 << P.H;
-    int S = P.S_cols * P.S_bot;
-    int64_t ntop = int64_t(1) << (P.logm - P.b0 - P.H);
-    P.n_bot_tiles = (P.bot_count + P.S_bot - 1) / P.S_bot;
-    P.n_col_tiles = (P.ncols + P.S_cols - 1) / P.S_cols;
-    int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
-    if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
-    size_t lds = (size_t)S * (W + 1) * 8;
-    hipLaunchKernelGGL(k_ntt_pass, dim3((unsigned)blocks), dim3(kThreads), lds, st, P);
-    check_hip(hipGetLastError(), "ntt pass launch");
-}
-
-}  // namespace
 
 // Executes a plan on a (and writes result); both [l, m, r, k] views.
 void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hipStream_t st) {
@@ -600,10 +191,11 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
     const int64_t inner_col_stride = A.cstride[0] < 0 ? -A.cstride[0] : A.cstride[0];
     const bool rows_contiguous = (a->strides[1] == 1) && inner_col_stride != 1;
 
-    // windows: one pass up to 12 bits, else windows of <= 8 bits
+    // windows: one pass up to 11 bits, else near-equal windows of <= 8 bits
+    // (12..16 bits: two register-radix-8 passes)
     const int logm = plan->logm;
     std::vector<int> win;
-    if (logm <= 12) {
+    if (logm <= 11) {
         win.push_back(logm);
     } else {
         int npass = (logm + 7) / 8;
@@ -677,11 +269,14 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         P.scale_out = last && plan->inverse;
         if (first && plan->inverse && plan->perm_kind == PERM_MAP) P.src_map = plan->rowmap;
         if (last && !plan->inverse && plan->perm_kind == PERM_MAP) P.dst_map = plan->rowmap;
+        // radix-8 passes hand lazy residues to a following radix-8 pass
+        P.lazy_out = !last && r8_window(H) && r8_window(win[ip + 1]);
         // tile shape
-        const int W = 1 << H;
-        int S = W >= 256 ? 8 : std::max(1, std::min(64, 2048 / W));
-        if (npass == 1 && W > 256) S = std::max(1, 4096 / W);
-        if (rows_contiguous && P.bot_count >= S) {
+        const bool col_tiles = !(rows_contiguous && P.bot_count >= 8);
+        int S = tile_subproblems(H, npass, col_tiles && ncols >= 16);
+        if (!col_tiles && P.bot_count < S && !r8_window(H))
+            S = std::max(1, (int)std::min<int64_t>(S, P.bot_count));
+        if (!col_tiles) {
             P.S_cols = 1;
             P.S_bot = S;
         } else {

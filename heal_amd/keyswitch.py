@@ -45,6 +45,21 @@ class KeySwitchBatch:
         # validate the Galois map once (synchronous, raises IndexError if out of range)
         capi.permute(self.coef.view(B * 2, N, L), self.perms, self.out.view(B * 2, N, L), 2, 1)
         self.graph = None
+        self.fused = None
+        self.key = None
+
+    # -- fused form: three kernels, key in device layout ----------------------
+    def prepare_fused(self):
+        if self.fused is None:
+            self.fused = capi.KeySwitchPlan(self.fwd, self.inv, self.perms, self.dnum, self.w)
+            self.key = self.fused.prepare_key(self.evk)
+        return self.fused
+
+    def run_fused(self, out=None):
+        self.prepare_fused()
+        out = self.out if out is None else out
+        self.fused(self.x, self.key, out)
+        return out
 
     # -- the six HEAL calls -------------------------------------------------
     def run(self):
@@ -60,16 +75,19 @@ class KeySwitchBatch:
         return self.out
 
     # -- hipGraph form --------------------------------------------------------
-    def capture(self):
+    def capture(self, fused: bool = False):
+        fn = self.run_fused if fused else self.run
+        if fused:
+            self.prepare_fused()
         stream = torch.cuda.Stream(device=self.device)
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
-            self.run()  # sizes the plan workspaces outside capture
+            fn()  # sizes the plan workspaces outside capture
         torch.cuda.current_stream(self.device).wait_stream(stream)
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=stream):
-            self.run()
+            fn()
         self.graph = g
         return g
 
@@ -78,8 +96,17 @@ class KeySwitchBatch:
         return self.out
 
     def close(self):
+        if self.fused is not None:
+            self.fused.close()
         self.fwd.close()
         self.inv.close()
+
+    def fused_bytes(self) -> int:
+        """HBM bytes the fused form moves per batch at its kernel boundaries:
+        read x, write/read the two intermediates, read the key, write out."""
+        P_ = 8 * self.n * self.L * self.B
+        D = self.dnum
+        return P_ + D * P_ + D * P_ + 2 * D * P_ + 2 * P_ + 2 * P_ + 2 * P_
 
     # -- bookkeeping ------------------------------------------------------------
     def algorithmic_bytes(self) -> int:

@@ -154,6 +154,26 @@ int hx_ntt_plan_destroy(hx_ntt_plan* plan);
 int hx_permute_ex(const hx_view* a, const hx_view* perms, const hx_view* result,
                   int64_t elementwise_axis, int64_t perm_axis, uint32_t flags, hx_stream stream);
 
+/* ------------------------------------------------- fused hybrid key-switch
+ * No counterpart in the reference (whose API composes a key-switch from six
+ * calls).  For B instances computes exactly
+ *   out = permute(intt(axis_modsum(modmul_ttt(ntt(g_decomposition(x)), evk), axis 0)), perms)
+ * with the views of that chain: x [B, m, k] int64, evk [dnum, B, 2, m, k],
+ * perms [k, m] (each row a permutation), out [B, 2, m, k]; fwd and inv are plans
+ * built from the same p and perm (inv with the inverse twiddles and m_inv).
+ * 2^12 <= m <= 2^16.  The key is laid out once for the device by
+ * hx_keyswitch_prepare_key into a caller-allocated buffer of
+ * hx_keyswitch_key_bytes(plan, B) bytes; exec then runs three kernels and never
+ * synchronises.  g_decomposition overflow is counted as by hx_g_decomposition. */
+typedef struct hx_keyswitch_plan hx_keyswitch_plan;
+int hx_keyswitch_plan_create(hx_keyswitch_plan** out, const hx_ntt_plan* fwd, const hx_ntt_plan* inv,
+                             const hx_view* perms, uint64_t dnum, uint64_t base_bits, hx_stream stream);
+int hx_keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch, size_t* bytes);
+int hx_keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hx_stream stream);
+int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                      const hx_view* out, hx_stream stream);
+int hx_keyswitch_plan_destroy(hx_keyswitch_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,105 @@ def test_keyswitch_graph_replays_match_eager(gpu):
     ks.close()
 
 
+@pytest.mark.parametrize("n,limbs,dnum,w,batch", [(4096, 3, 3, 20, 2), (8192, 5, 2, 30, 1), (16384, 4, 4, 15, 2),
+                                                   (32768, 2, 1, 60, 1), (65536, 3, 3, 60, 1)])
+def test_fused_keyswitch_vs_oracle(gpu, n, limbs, dnum, w, batch):
+    from heal_amd import capi
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    print(f"[fused] n={n} L={limbs} D={dnum} w={w} B={batch}", flush=True)
+    ks = KeySwitchBatch(n=n, limbs=limbs, dnum=dnum, base_bits=w, batch=batch, seed=n + dnum)
+    if n == 8192:
+        # unreduced key entries and unreduced / negative x exercise the chain's C++ integer semantics
+        # (g_decomposition's unsigned digits, modmul's reduction of unreduced operands)
+        ks.evk[:, :, :, ::7] += 1 << 62
+        ks.x[:, ::5] = ks.x[:, ::5] * 3 + (1 << 61)
+        ks.x[:, 1::9] = -ks.x[:, 1::9]
+    capi.status_poll()
+    out = torch.zeros_like(ks.out)
+    ks.run_fused(out)
+    st = capi.status_poll()
+    for b in range(batch):
+        assert torch.equal(out[b].cpu(), oracle_keyswitch(ks, b)), b
+    # the g_decomposition overflow count matches the per-call op
+    ks.run()
+    st2 = capi.status_poll()
+    assert st["gdecomp_overflow"] == st2["gdecomp_overflow"]
+    assert torch.equal(out, ks.out)
+    ks.close()
+
+
+def test_fused_keyswitch_negative_key_matches_chain(gpu):
+    """Negative key entries give signed products and digit sums (C++ truncation);
+    the INTT then reduces its negative inputs mathematically, in both forms."""
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    ks = KeySwitchBatch(n=4096, limbs=3, dnum=3, base_bits=20, batch=2, seed=77)
+    ks.evk[:, :, :, ::3] = -ks.evk[:, :, :, ::3]
+    ks.run()
+    torch.cuda.synchronize()
+    want = ks.out.clone()
+    out = torch.zeros_like(want)
+    ks.run_fused(out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, want)
+    ks.close()
+
+
+def test_fused_keyswitch_full_size_matches_plain_chain(gpu):
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    ks = KeySwitchBatch(n=65536, limbs=24, dnum=3, base_bits=20, batch=2, seed=5)
+    ks.run()
+    torch.cuda.synchronize()
+    want = ks.out.clone()
+    out = torch.zeros_like(ks.out)
+    ks.run_fused(out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, want)
+    # graph replays of the fused form, with eager work in between
+    ks.capture(fused=True)
+    for i in range(3):
+        ks.out.zero_()
+        ks.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ks.out, want), i
+        ks.run_fused(out)
+        torch.cuda.synchronize()
+        assert torch.equal(out, want), i
+    ks.close()
+
+
+def test_fused_keyswitch_errors(gpu):
+    from heal_amd import capi, params as P
+
+    n, L = 4096, 2
+    primes = P.keyswitch_primes(L, n)
+    prm = P.NttParams(n, primes, device="cuda")
+    fwd = capi.NttPlan(prm.p, prm.perm, prm.twiddles)
+    inv = capi.NttPlan(prm.p, prm.perm, prm.inv_twiddles, prm.m_inv, inverse=True)
+    gal = P.galois_perms(n, 5, L, device="cuda")
+    with pytest.raises(ValueError):  # two forward plans
+        capi.KeySwitchPlan(fwd, fwd, gal, 3, 20)
+    other = capi.NttPlan(prm.p, torch.arange(n, device="cuda"), prm.inv_twiddles, prm.m_inv, inverse=True)
+    with pytest.raises(ValueError):  # different permutations
+        capi.KeySwitchPlan(fwd, other, gal, 3, 20)
+    bad = gal.clone()
+    bad[1, 3] = bad[1, 4]
+    with pytest.raises(ValueError):  # a row that is not a permutation
+        capi.KeySwitchPlan(fwd, inv, bad, 3, 20)
+    bad[1, 3] = n
+    with pytest.raises(IndexError):
+        capi.KeySwitchPlan(fwd, inv, bad, 3, 20)
+    small = P.NttParams(1024, P.keyswitch_primes(L, 1024), device="cuda")
+    f2 = capi.NttPlan(small.p, small.perm, small.twiddles)
+    i2 = capi.NttPlan(small.p, small.perm, small.inv_twiddles, small.m_inv, inverse=True)
+    with pytest.raises(ValueError):  # m below 2^12
+        capi.KeySwitchPlan(f2, i2, P.galois_perms(1024, 5, L, device="cuda"), 3, 20)
+    for p in (fwd, inv, other, f2, i2):
+        p.close()
+
+
 @pytest.mark.slow
 def test_keyswitch_full_size_vs_oracle(gpu):
     """Config: N = 2^16, L = 24 primes of 50-60 bits, dnum = 3; w = 60 (as specified) and w = 20 (all digits live)."""
