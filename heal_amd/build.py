@@ -51,6 +51,55 @@ def _compile(src: str) -> str:
     return obj
 
 
+API_DIR = os.path.join(HERE, "api")
+API_LIB = os.path.join(HERE, "libheal_lattica.so")
+
+
+def _torch_flags():
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+
+    inc = [f"-I{p}" for p in ce.include_paths()] + [f"-I{sysconfig.get_paths()['include']}"]
+    libdirs = ce.library_paths()
+    abi = int(torch.compiled_with_cxx11_abi())
+    cflags = ["-O2", "-std=c++17", "-fPIC", f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-w", *inc]
+    ldflags = [f"-L{d}" for d in libdirs] + [f"-Wl,-rpath,{d}" for d in libdirs]
+    return cflags, ldflags
+
+
+def _module_path() -> str:
+    import sysconfig
+
+    return os.path.join(HERE, "lattica_hw" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_api() -> list[str]:
+    """The C++ HEAL API templates (libheal_lattica.so) and the Python module lattica_hw."""
+    os.makedirs(BUILD, exist_ok=True)
+    cflags, ldflags = _torch_flags()
+    api_obj = os.path.join(BUILD, "lattica_hw_api.o")
+    bind_obj = os.path.join(BUILD, "py_bindings.o")
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        jobs = [
+            ex.submit(_run, ["g++", *cflags, "-c", os.path.join(API_DIR, "lattica_hw_api.cpp"), "-o", api_obj]),
+            ex.submit(_run, ["g++", *cflags, "-DTORCH_API_INCLUDE_EXTENSION_H", "-DTORCH_EXTENSION_NAME=lattica_hw",
+                             "-c", os.path.join(API_DIR, "py_bindings.cpp"), "-o", bind_obj]),
+        ]
+        for j in jobs:
+            j.result()
+    origin = "-Wl,-rpath,$ORIGIN"
+    _run(["g++", "-shared", "-o", API_LIB + ".tmp", api_obj, f"-L{HERE}", "-lheal_amd", origin, *ldflags,
+          "-ltorch", "-ltorch_cpu", "-lc10"])
+    os.replace(API_LIB + ".tmp", API_LIB)
+    mod = _module_path()
+    _run(["g++", "-shared", "-o", mod + ".tmp", bind_obj, f"-L{HERE}", "-lheal_lattica", "-lheal_amd", origin,
+          *ldflags, "-ltorch", "-ltorch_cpu", "-lc10", "-ltorch_python"])
+    os.replace(mod + ".tmp", mod)
+    return [API_LIB, mod]
+
+
 def build_all(jobs: int = 8) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -59,6 +108,7 @@ def build_all(jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_api()
     return LIB
 
 
