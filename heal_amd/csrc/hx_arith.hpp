@@ -148,6 +148,20 @@ HX_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
     return x * w - qh * q;
 }
 
+// shoup_lazy with nq = 2^64 - q: x*w + qh*nq = x*w - qh*q (mod 2^64), written
+// over 32-bit halves so that the two low products share one 64-bit
+// multiply-add chain and the cross terms one high word.
+HX_HD uint64_t shoup_lazy_nq(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) {
+    const uint64_t qh = mulhi64(x, wp);
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+    const uint32_t wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
+    const uint32_t ql = (uint32_t)qh, qhh = (uint32_t)(qh >> 32);
+    const uint32_t nl = (uint32_t)nq, nh = (uint32_t)(nq >> 32);
+    const uint64_t p = (uint64_t)xl * wl + (uint64_t)ql * nl;  // mod 2^64
+    const uint32_t hi = (uint32_t)(p >> 32) + xl * wh + xh * wl + ql * nh + qhh * nl;
+    return ((uint64_t)hi << 32) | (uint32_t)p;
+}
+
 HX_HD uint64_t shoup_mul(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
     uint64_t r = shoup_lazy(x, w, wp, q);
     return r >= q ? r - q : r;

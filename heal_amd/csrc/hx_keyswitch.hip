@@ -57,17 +57,25 @@ __device__ __forceinline__ int64_t seq_rem(const ModInfo& mi, int64_t s) {
     return cpp_rem(mi, s);
 }
 
+// K2 on the radix-16 register layouts (see k_ntt_pass_r16).  A workgroup
+// takes the D digit columns of G limbs of one instance and one value `hi` of
+// the high index bits (rows hi*W .. hi*W + W - 1):
+//   forward  sub-problem s = d + D*gi: load in layout A, stages, LDS
+//            exchange, stages in layout B, full reduction, values to LDS
+//            in natural order;
+//   inverse  sub-problem s = c + 2*gi: the key inner product for the values
+//            of layout B, stages, LDS exchange, stages in layout A, store.
+// The prepared key holds window index wB(e, lane) of row block hi at
+// hi*W + e*TL + lane, so a key load of one register covers TL adjacent words.
 template <int H>
-__global__ __launch_bounds__(1024) void k_ks_inner(InnerArgs A) {
+__global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
     constexpr int W = 1 << H;
-    constexpr int WP = W + W / 32;
-    constexpr int TL = W / 8;
-    extern __shared__ uint64_t lds[];
+    constexpr int TL = W / 16;
+    constexpr int SP = W + 1;
+    extern __shared__ uint64_t lds[];  // [max(NF, NI)][SP]
     const int D = A.D, G = A.G;
-    const int NF = G * D;  // forward sub-problems: s = d + D * gi
-    const int NI = 2 * G;  // inverse sub-problems: s = c + 2 * gi
-    uint64_t* xf = lds;
-    uint64_t* xi = lds + NF * WP;
+    const int NF = G * D, NI = 2 * G;
+    const int tid = threadIdx.x;
 
     int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t hi = tile % A.n_hi;
@@ -78,100 +86,128 @@ __global__ __launch_bounds__(1024) void k_ks_inner(InnerArgs A) {
     const int64_t ncols1 = (int64_t)D * A.k * A.B, ncols2 = 2 * A.k * A.B;
     const int64_t row0 = hi * W;
 
-    // ---- load the D digit columns of each limb of the group (adjacent in memory)
-    {
-        const int s = threadIdx.x % NF, step = blockDim.x / NF;
-        const int64_t t = t0 + s / D;
-        if (t < A.k) {
-            const uint64_t* src = A.src + row0 * ncols1 + (s % D) + (int64_t)D * (t + A.k * b);
-            for (int w = threadIdx.x / NF; w < W; w += step) xf[s * WP + r8_pad(w)] = src[(int64_t)w * ncols1];
-        }
-    }
-    __syncthreads();
-
     // ---- forward stages over the low bits (global stages hhi .. logm-1)
     {
-        const int s = threadIdx.x / TL, lane = threadIdx.x % TL;
-        const bool active = s < NF;
-        const int ss = active ? s : 0;
-        int64_t t = t0 + ss / D;
-        if (t >= A.k) t = A.k - 1;  // a partial last group computes on stale values, never stored
+        const bool act = tid < NF * TL;
+        const int s = act ? tid % NF : 0, lane = act ? tid / NF : 0;
+        int64_t t = t0 + s / D;
+        const bool valid = act && t < A.k;
+        if (t >= A.k) t = A.k - 1;
         const LimbInfo& L = A.limbs[t];
-        TwRow R;
-        R.tw = (const ulonglong2*)(A.tw_f + 2 * t * A.m);
-        R.top = hi;
-        R.logm = A.logm;
-        R.g0 = A.hhi;
-        r8_all_rounds<H, false>(xf + ss * WP, lane, R, L.q, limb_mode(L), active);
+        const int mode = limb_mode(L);
+        const ulonglong2* twrow = (const ulonglong2*)(A.tw_f + 2 * t * A.m);
+        uint64_t x[16] = {};
+        if (valid) {
+            const uint64_t* src = A.src + (row0 + lane) * ncols1 + (s % D) + (int64_t)D * (t + A.k * b);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = src[(int64_t)(e * TL) * ncols1];
+            r16_stages_any<H, false, 0>(x, lane, twrow, hi, A.logm, A.hhi, L.q, mode);
+        }
+        uint64_t* xs = lds + s * SP;
+        if (act) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xs[r16_wA<H>(e, lane)] = x[e];
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = xs[r16_wB<H>(e, lane)];
+            r16_stages_any<H, false, 1>(x, lane, twrow, hi, A.logm, A.hhi, L.q, mode);
+            if (mode == MODE_L64) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], L.q);
+            }
+            // each thread rewrites exactly the slots it has just read
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xs[r16_wB<H>(e, lane)] = x[e];
+        }
+        __syncthreads();
     }
 
     // ---- key inner product and sum over digits, in the chain's integer semantics:
     //      prod_d = (T_DP(D_d) * T_DP(key_d)) % q      (modmul_ttt)
     //      sum    = (sum + prod_d) % q, d = 0..D-1     (axis_modsum)
     //      INTT input = residue of sum                  (intt load)
-    for (int e = threadIdx.x; e < G * W; e += blockDim.x) {
-        const int gi = e / W, w = e % W;
-        const int64_t t = t0 + gi;
-        if (t >= A.k) continue;
-        const LimbInfo& L = A.limbs[t];
+    // For key words in [0, q) this is (sum_d D_d * key_d) mod q, accumulated in
+    // 128 bits and reduced once when D * q < 2^64; other key words take the
+    // exact step-by-step path.
+    const bool act = tid < NI * TL;
+    const int s = act ? tid % NI : 0, lane = act ? tid / NI : 0;
+    const int c = s & 1, gi = s >> 1;
+    int64_t t = t0 + gi;
+    const bool valid = act && t < A.k;
+    if (t >= A.k) t = A.k - 1;
+    const LimbInfo& L = A.limbs[t];
+    const uint64_t q = L.q;
+    const int mode = limb_mode(L);
+    uint64_t x[16] = {};
+    if (valid) {
         const ModInfo mi = limb_modinfo(L);
-        const bool lazy = limb_mode(L) == MODE_L64;
-        const int64_t j = row0 + w;
+        const bool one_reduction = (uint64_t)D <= ~0ull / q;
+        const int64_t* kp = A.key + ((b * 2 + c) * A.k + t) * A.m + row0 + lane;
+        const int64_t dstride = A.B * 2 * A.k * A.m;
+        const uint64_t* xf = lds + (int64_t)(D * gi) * SP;
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            int64_t sum = 0;
+        for (int e = 0; e < 16; ++e) {
+            const int w = r16_wB<H>(e, lane);
+            u128 acc = 0;
+            bool canon = one_reduction;
             for (int d = 0; d < D; ++d) {
-                const int64_t kv = A.key[((((int64_t)d * A.B + b) * 2 + c) * A.k + t) * A.m + j];
-                const uint64_t v = xf[(d + D * gi) * WP + r8_pad(w)];
-                const uint64_t x = lazy ? lazy_reduce(v, L.q) : v;
-                int64_t prod;
-                if (kv >= 0 && (uint64_t)kv < L.q) prod = (int64_t)mulmod(mi, x, (uint64_t)kv);
-                else prod = cpp_mulmod(mi, (int64_t)x, kv);
-                sum = seq_rem(mi, (int64_t)((uint64_t)sum + (uint64_t)prod));
+                const int64_t kv = kp[d * dstride + e * TL];
+                canon = canon && kv >= 0 && (uint64_t)kv < q;
+                acc += (u128)xf[d * SP + w] * (uint64_t)kv;
             }
-            uint64_t v = (uint64_t)sum;
-            if (sum < 0) {
-                uint64_t r = reduce64(mi, uabs64(sum));
-                v = r ? L.q - r : 0;
+            if (canon) {
+                x[e] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
+            } else {
+                int64_t sum = 0;
+                for (int d = 0; d < D; ++d) {
+                    const int64_t kv = kp[d * dstride + e * TL];
+                    const uint64_t xv = xf[d * SP + w];
+                    const int64_t prod = (kv >= 0 && (uint64_t)kv < q) ? (int64_t)mulmod(mi, xv, (uint64_t)kv)
+                                                                       : cpp_mulmod(mi, (int64_t)xv, kv);
+                    sum = seq_rem(mi, (int64_t)((uint64_t)sum + (uint64_t)prod));
+                }
+                uint64_t v = (uint64_t)sum;
+                if (sum < 0) {
+                    const uint64_t r = reduce64(mi, uabs64(sum));
+                    v = r ? q - r : 0;
+                }
+                x[e] = v;
             }
-            xi[(c + 2 * gi) * WP + r8_pad(w)] = v;
         }
     }
-    __syncthreads();
 
     // ---- inverse stages over the low bits (global stages 0 .. hlo-1)
-    {
-        const int s = threadIdx.x / TL, lane = threadIdx.x % TL;
-        const bool active = s < NI;
-        const int ss = active ? s : 0;
-        int64_t t = t0 + ss / 2;
-        if (t >= A.k) t = A.k - 1;
-        const LimbInfo& L = A.limbs[t];
-        TwRow R;
-        R.tw = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
-        R.top = hi;
-        R.logm = A.logm;
-        R.g0 = 0;
-        r8_all_rounds<H, true>(xi + ss * WP, lane, R, L.q, limb_mode(L), active);
+    const ulonglong2* twrow = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
+    if (valid) r16_stages_any<H, true, 1>(x, lane, twrow, hi, A.logm, 0, q, mode);
+    __syncthreads();  // every read of the forward values is done
+    uint64_t* xs = lds + s * SP;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) xs[r16_wB<H>(e, lane)] = x[e];
     }
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = xs[r16_wA<H>(e, lane)];
+    r16_stages_any<H, true, 0>(x, lane, twrow, hi, A.logm, 0, q, mode);
 
-    // ---- store (residues stay lazy for the next radix-8 pass)
-    {
-        const int s = threadIdx.x % NI, step = blockDim.x / NI;
-        const int64_t t = t0 + s / 2;
-        if (t >= A.k) return;
-        uint64_t* dst = A.dst + row0 * ncols2 + (s % 2) + 2 * (t + A.k * b);
-        for (int w = threadIdx.x / NI; w < W; w += step) dst[(int64_t)w * ncols2] = xi[s * WP + r8_pad(w)];
-    }
+    // ---- store (residues stay lazy for the next radix-16 pass)
+    uint64_t* dst = A.dst + (row0 + lane) * ncols2 + c + 2 * (t + A.k * b);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dst[(int64_t)(e * TL) * ncols2] = x[e];
 }
 
-// key[((d*B + b)*2 + c)*k + t][j] = evk[d, b, c, rowmap[j], t]
+// key[((d*B + b)*2 + c)*k + t][hi*W + e*TL + lane] = evk[d, b, c, rowmap[j], t]
+// with j = hi*W + r16_wB(e, lane), W = 2^H the inner window (H = 6..8)
 struct KeyArgs {
     const int64_t* evk;
     int64_t s[5];  // strides of the [D, B, 2, m, k] view
     int64_t* key;
     const int32_t* rowmap;  // null: identity
     int64_t D, B, m, k;
+    int32_t H;
 };
 
 __global__ void k_ks_prepare_key(KeyArgs K) {
@@ -183,7 +219,13 @@ __global__ void k_ks_prepare_key(KeyArgs K) {
         int64_t c = r % 2;
         r /= 2;
         int64_t b = r % K.B, d = r / K.B;
-        int64_t u = K.rowmap ? (int64_t)K.rowmap[j] : j;
+        // position j inside its window block -> window index r16_wB(e, lane)
+        const int nb = K.H - 4, tl = 1 << nb;
+        const int64_t wmask = (int64_t(1) << K.H) - 1;
+        const int pos = (int)(j & wmask), reg = pos / tl, lane = pos % tl;
+        const int w = ((reg >> nb) << (2 * nb)) | (lane << nb) | (reg & ((1 << nb) - 1));
+        const int64_t jw = (j & ~wmask) | w;
+        int64_t u = K.rowmap ? (int64_t)K.rowmap[jw] : jw;
         K.key[e] = K.evk[d * K.s[0] + b * K.s[1] + c * K.s[2] + u * K.s[3] + t * K.s[4]];
     }
 }
@@ -330,6 +372,7 @@ void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, vo
     K.B = evk->dims[1];
     K.m = plan->m;
     K.k = plan->k;
+    K.H = plan->hlo;
     hipLaunchKernelGGL(k_ks_prepare_key, dim3(grid_for(K.D * K.B * 2 * K.k * K.m, 256, 8192)), dim3(256), 0, st, K);
     check_hip(hipGetLastError(), "prepare key launch");
 }
@@ -352,6 +395,7 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     if (dev != plan->device) raise(HX_ERR_INVALID_ARGUMENT, "plan belongs to another device");
     if (B == 0) return;
+    if (!r16_window(plan->hhi)) raise(HX_ERR_RUNTIME, "the fused key-switch needs the radix-16 pass kernels");
     const int64_t ncols1 = (int64_t)D * k * B, ncols2 = 2 * k * B;
     const size_t bytes = (size_t)m * (size_t)(ncols1 + ncols2) * 8;
     if (plan->ws_bytes < bytes) {
@@ -400,8 +444,7 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     P1.limbs = F->limbs;
     P1.tw = F->tw;
     P1.m = m;
-    P1.S_cols = tile_subproblems(P1.H, 2, ncols1 >= 16);
-    P1.S_bot = 1;
+    configure_tile(P1, true, 2);
     launch_pass(P1, st);
 
     // K2: forward low bits, key inner product, inverse low bits
@@ -425,10 +468,11 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     A.n_tg = (k + G - 1) / G;
     A.n_hi = int64_t(1) << plan->hhi;
     const int H = plan->hlo;
-    const int TL = (1 << H) / 8;
-    const int threads = std::max(G * D, 2 * G) * TL;
-    const size_t lds = (size_t)(G * D + 2 * G) * ((1 << H) + (1 << H) / 32) * 8;
-    if (threads > 1024 || lds > 64 * 1024)
+    const int TL = (1 << H) / 16;
+    const int rows = std::max(G * D, 2 * G);
+    const int threads = (rows * TL + 63) / 64 * 64;
+    const size_t lds = (size_t)rows * ((1 << H) + 1) * 8;
+    if (threads > 512 || lds > 64 * 1024)
         raise(HX_ERR_INVALID_ARGUMENT, "dnum too large for the fused key-switch tile");
     const int64_t blocks = A.n_hi * A.n_tg * B;
     if (H == 8) hipLaunchKernelGGL(k_ks_inner<8>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
@@ -462,8 +506,7 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     P3.limbs = I->limbs;
     P3.tw = I->tw;
     P3.m = m;
-    P3.S_cols = tile_subproblems(P3.H, 2, ncols2 >= 16);
-    P3.S_bot = 1;
+    configure_tile(P3, true, 2);
     launch_pass(P3, st);
 }
 

@@ -11,11 +11,12 @@
 // network.
 //
 // Schedule: the log2(m) stages are cut into windows of at most 8 bits (or one
-// window of up to 12 bits when m <= 4096).  One launch per window ("pass")
-// loads, for S sub-transforms at a time, the 2^H elements that the window's
-// stages couple into LDS, runs the H stages there with one barrier per stage,
-// and writes back.  Forward windows go from the top index bits down, inverse
-// windows from the bottom up.  The permutation is fused into the pass that
+// window of up to 11 bits when m <= 2048).  One launch per window ("pass")
+// handles, for S sub-transforms at a time, the 2^H elements that the window's
+// stages couple: windows of 6..8 bits run in registers, 16 values per lane,
+// with one LDS exchange (k_ntt_pass_r16); other windows run in LDS with one
+// barrier per stage (k_ntt_pass).  Forward windows go from the top index bits
+// down, inverse windows from the bottom up.  The permutation is fused into the pass that
 // touches HBM last (forward) or first (inverse) as a row map.  Butterflies use
 // Shoup multiplication with companions precomputed once per plan; limbs with
 // q < 2^31 use 32-bit arithmetic.
@@ -192,7 +193,7 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
     const bool rows_contiguous = (a->strides[1] == 1) && inner_col_stride != 1;
 
     // windows: one pass up to 11 bits, else near-equal windows of <= 8 bits
-    // (12..16 bits: two register-radix-8 passes)
+    // (12..16 bits: two register-radix-16 passes)
     const int logm = plan->logm;
     std::vector<int> win;
     if (logm <= 11) {
@@ -269,20 +270,9 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         P.scale_out = last && plan->inverse;
         if (first && plan->inverse && plan->perm_kind == PERM_MAP) P.src_map = plan->rowmap;
         if (last && !plan->inverse && plan->perm_kind == PERM_MAP) P.dst_map = plan->rowmap;
-        // radix-8 passes hand lazy residues to a following radix-8 pass
-        P.lazy_out = !last && r8_window(H) && r8_window(win[ip + 1]);
-        // tile shape
-        const bool col_tiles = !(rows_contiguous && P.bot_count >= 8);
-        int S = tile_subproblems(H, npass, col_tiles && ncols >= 16);
-        if (!col_tiles && P.bot_count < S && !r8_window(H))
-            S = std::max(1, (int)std::min<int64_t>(S, P.bot_count));
-        if (!col_tiles) {
-            P.S_cols = 1;
-            P.S_bot = S;
-        } else {
-            P.S_cols = S;
-            P.S_bot = 1;
-        }
+        // radix-16 passes hand lazy residues to a following radix-16 pass
+        P.lazy_out = !last && r16_window(H) && r16_window(win[ip + 1]);
+        configure_tile(P, !(rows_contiguous && P.bot_count >= 8), npass);
         launch_pass(P, st);
         done_bits += H;
     }

@@ -1,5 +1,5 @@
 // hx_ntt_kernels.hpp — NTT device code shared by hx_ntt.hip and hx_keyswitch.hip:
-// plan layout, pass arguments, the generic and register-radix-8 pass kernels,
+// plan layout, pass arguments, the generic and register-radix-16 pass kernels,
 // and the host helpers that configure and launch a pass.  Everything below
 // lives in an anonymous namespace, so each translation unit that includes
 // this header gets its own copy of the kernels.
@@ -186,7 +186,7 @@ struct PassArgs {
     int64_t n_bot_tiles, n_col_tiles;
     int32_t reduce_in;
     int32_t scale_out;
-    int32_t lazy_out;        // radix-8 passes: leave 64-bit residues in [0, 4q) for the next radix-8 pass
+    int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 4q) forward, [0, 2q) inverse)
     // digit source (src.kind == 3): column axis `gd_axis` is the digit index d,
     // the value loaded is digit d of x in base 2^gd_w; elements of digit 0 whose
     // value is not exhausted after gd_D digits are counted in gd_status
@@ -378,16 +378,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs P) {
 }
 
 // ---------------------------------------------------------------------------
-// Register-radix-8 pass for windows of H = 6..8 bits.
-//
-// A sub-problem (2^H points) is held by 2^(H-3) lanes with 8 values each.  The
-// window's stages run in three rounds of at most 3 stages (bits [H-3, H),
-// [H-6, H-3), [0, H-6)); inside a round every butterfly is between two
-// registers of one lane, and the rounds exchange values through LDS.  In a
-// round over bits [r0, r0 + nb), value e of lane `lane` is window index
-//     w = (rest >> r0) << (r0 + nb) | (e mod 2^nb) << r0 | (rest mod 2^r0),
-//     rest = (e >> nb) << (H - 3) | lane.
-// LDS slots are padded by one every 32 entries to spread banks.
+// Butterfly arithmetic of the register-radix-16 passes.
 //
 // Arithmetic per limb (uniform over a sub-problem):
 //   MODE_S32  q < 2^31: 32-bit Shoup, residues kept in [0, q);
@@ -401,21 +392,6 @@ enum : int { MODE_S32 = 0, MODE_L64 = 1, MODE_F64 = 2 };
 __device__ __forceinline__ int limb_mode(const LimbInfo& L) {
     return L.small ? MODE_S32 : (L.q < (uint64_t(1) << 62) ? MODE_L64 : MODE_F64);
 }
-
-template <int H>
-__device__ __forceinline__ int r8_w(int lane, int e, int r0, int nb) {
-    int v = e & ((1 << nb) - 1);
-    int rest = ((e >> nb) << (H - 3)) | lane;
-    return ((rest >> r0) << (r0 + nb)) | (v << r0) | (rest & ((1 << r0) - 1));
-}
-
-__device__ __forceinline__ int r8_pad(int w) { return w + (w >> 5); }
-
-struct TwRow {
-    const ulonglong2* tw;  // (w, companion) pairs of this limb
-    int64_t top;
-    int logm, g0;
-};
 
 template <int MODE, bool INV>
 __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w, uint64_t wp, uint64_t q) {
@@ -459,68 +435,6 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
     }
 }
 
-// One round (nb stages) on the 8 registers of a lane.  The twiddle index of a
-// butterfly depends on the bits of its lower element above the stage's window
-// bit; writing it as (rest_high, round bits above bi) lets equal indices
-// within a lane fold into one load.
-template <int H, bool INV, int MODE>
-__device__ __forceinline__ void r8_round(uint64_t (&x)[8], int lane, int r0, int nb, const TwRow& R, uint64_t q) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nb) break;
-        const int bi = INV ? k : (nb - 1 - k);  // bit of e
-        const int wbit = r0 + bi;               // window bit
-        const int st = INV ? wbit : (H - 1 - wbit);
-        const int g = R.g0 + st;
-        const int64_t base = INV ? ((int64_t(1) << (R.logm - 1 - g)) + (R.top << (H - 1 - st)))
-                                 : ((int64_t(1) << g) + (R.top << st));
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            if (e & (1 << bi)) continue;
-            const int e2 = e | (1 << bi);
-            // (w_lo >> (wbit + 1)) = rest_high << (nb - bi - 1) | (v >> (bi + 1))
-            const int rest_high = (((e >> nb) << (H - 3)) | lane) >> r0;
-            const int hi = (rest_high << (nb - bi - 1)) | ((e & ((1 << nb) - 1)) >> (bi + 1));
-            ulonglong2 tw = R.tw[base + hi];
-            butterfly<MODE, INV>(x[e], x[e2], tw.x, tw.y, q);
-        }
-    }
-}
-
-template <int H, bool INV, int MODE>
-__device__ __forceinline__ void r8_round_lds(uint64_t* xs_s, int lane, int r0, int nb, const TwRow& R, uint64_t q) {
-    uint64_t x[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = xs_s[r8_pad(r8_w<H>(lane, e, r0, nb))];
-    r8_round<H, INV, MODE>(x, lane, r0, nb, R, q);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) xs_s[r8_pad(r8_w<H>(lane, e, r0, nb))] = x[e];
-}
-
-// All rounds of one sub-problem.  Every thread of the workgroup reaches the
-// same barriers; only the arithmetic between them depends on the limb mode.
-// Threads with active == false (no sub-problem of their own) only take part
-// in the barriers.
-template <int H, bool INV>
-__device__ __forceinline__ void r8_all_rounds(uint64_t* xs_s, int lane, const TwRow& R, uint64_t q, int mode,
-                                              bool active = true) {
-    constexpr int nC = H - 6;
-    // forward: rounds A (top bits), B, C; inverse: C, B, A
-#pragma unroll
-    for (int ri = 0; ri < 3; ++ri) {
-        const int r = INV ? 2 - ri : ri;
-        const int r0 = r == 0 ? H - 3 : (r == 1 ? H - 6 : 0);
-        const int nb = r == 2 ? nC : 3;
-        if (nb == 0) continue;
-        if (active) {
-            if (mode == MODE_L64) r8_round_lds<H, INV, MODE_L64>(xs_s, lane, r0, nb, R, q);
-            else if (mode == MODE_S32) r8_round_lds<H, INV, MODE_S32>(xs_s, lane, r0, nb, R, q);
-            else r8_round_lds<H, INV, MODE_F64>(xs_s, lane, r0, nb, R, q);
-        }
-        __syncthreads();
-    }
-}
-
 // Fully reduce a lazy value of a MODE_L64 limb.
 __device__ __forceinline__ uint64_t lazy_reduce(uint64_t x, uint64_t q) {
     const uint64_t q2 = q << 1;
@@ -528,112 +442,218 @@ __device__ __forceinline__ uint64_t lazy_reduce(uint64_t x, uint64_t q) {
     return x >= q ? x - q : x;
 }
 
-template <int H, bool INV, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_ntt_pass_r8(PassArgs P) {
+// ---------------------------------------------------------------------------
+// Register-radix-16 pass for windows of H = 6..8 bits (the NTT passes).
+//
+// A sub-problem (2^H points) is held by TL = 2^nb lanes (nb = H - 4) with 16
+// values each, in one of two layouts:
+//   layout A, window bits [nb, H) inside a lane:  w = e << nb | lane
+//   layout B, window bits [0, nb) inside a lane:  w = (e >> nb) << 2nb | lane << nb | (e mod 2^nb)
+// The forward pass loads straight from global memory into layout A, runs the
+// four stages over the top window bits, moves the values through LDS into
+// layout B, runs the remaining stages and stores from registers; the inverse
+// pass does the same in the opposite order.  Thread t serves sub-problem
+// t mod S and lane t / S, so a load instruction covers S adjacent columns (or
+// rows) of the tile, and 32 consecutive threads address 32 different LDS
+// rows of odd length (no bank conflicts in either layout).
+template <int H>
+__device__ __forceinline__ int r16_wA(int e, int lane) {
+    return (e << (H - 4)) | lane;
+}
+
+template <int H>
+__device__ __forceinline__ int r16_wB(int e, int lane) {
+    constexpr int nb = H - 4;
+    return ((e >> nb) << (2 * nb)) | (lane << nb) | (e & ((1 << nb) - 1));
+}
+
+// The stages of one layout on the 16 registers of a lane.  The twiddle index
+// of a butterfly is ubase(stage) + (w >> (b + 1)) for window bit b; in layout
+// A that is ubase + (e >> (b - nb + 1)), in layout B ubase + lane part + a
+// constant, so each stage needs one address and immediate offsets.
+template <int H, bool INV, int MODE, int LAYOUT>
+__device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ulonglong2* twrow, int64_t top,
+                                           int logm, int g0, uint64_t q) {
+    constexpr int nb = H - 4;
+    constexpr int nst = LAYOUT == 0 ? 4 : nb;
+#pragma unroll
+    for (int k = 0; k < nst; ++k) {
+        // forward: high window bits first; inverse: low bits first
+        const int b = LAYOUT == 0 ? (INV ? nb + k : H - 1 - k) : (INV ? k : nb - 1 - k);
+        const int eb = LAYOUT == 0 ? b - nb : b;  // the bit of e that b is
+        const int st = INV ? b : H - 1 - b;       // stage within the window
+        const int g = g0 + st;                    // global stage
+        const int64_t ubase = INV ? ((int64_t(1) << (logm - 1 - g)) + (top << (H - 1 - b)))
+                                  : ((int64_t(1) << g) + (top << st));
+        const ulonglong2* tp = twrow + ubase + (LAYOUT == 1 ? ((int64_t)lane << (nb - b - 1)) : 0);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (e & (1 << eb)) continue;
+            const int e2 = e | (1 << eb);
+            const int c = LAYOUT == 0 ? (e >> (b - nb + 1))
+                                      : (((e >> nb) << (2 * nb - b - 1)) | ((e & ((1 << nb) - 1)) >> (b + 1)));
+            const ulonglong2 tw = tp[c];
+            butterfly<MODE, INV>(x[e], x[e2], tw.x, tw.y, q);
+        }
+    }
+}
+
+template <int H, bool INV, int LAYOUT>
+__device__ __forceinline__ void r16_stages_any(uint64_t (&x)[16], int lane, const ulonglong2* twrow, int64_t top,
+                                               int logm, int g0, uint64_t q, int mode) {
+    if (mode == MODE_L64) r16_stages<H, INV, MODE_L64, LAYOUT>(x, lane, twrow, top, logm, g0, q);
+    else if (mode == MODE_S32) r16_stages<H, INV, MODE_S32, LAYOUT>(x, lane, twrow, top, logm, g0, q);
+    else r16_stages<H, INV, MODE_F64, LAYOUT>(x, lane, twrow, top, logm, g0, q);
+}
+
+template <int H>
+__device__ __forceinline__ int r16_w(int layout, int e, int lane) {
+    return layout == 0 ? r16_wA<H>(e, lane) : r16_wB<H>(e, lane);
+}
+
+// w = r16_wl(layout, lane) + r16_wc(layout, e): the per-lane part of a window
+// index and the part fixed by the register number, so that an address is one
+// per-lane base plus a uniform offset per register.
+template <int H>
+__device__ __forceinline__ int r16_wl(int layout, int lane) {
+    return layout == 0 ? lane : lane << (H - 4);
+}
+
+template <int H>
+__device__ __forceinline__ int r16_wc(int layout, int e) {
+    constexpr int nb = H - 4;
+    return layout == 0 ? e << nb : ((e >> nb) << (2 * nb)) + (e & ((1 << nb) - 1));
+}
+
+template <int H, bool INV>
+__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
     constexpr int W = 1 << H;
-    constexpr int WP = W + W / 32;
-    constexpr int TL = W / 8;          // lanes per sub-problem
-    constexpr int S = THREADS / TL;    // sub-problems per workgroup
-    extern __shared__ uint64_t xs[];   // [S][WP]
+    constexpr int S = 32;               // sub-problems per workgroup
+    constexpr int SP = W + 1;           // LDS row length (odd)
+    constexpr int L_IN = INV ? 1 : 0;   // layout of the first stages
+    constexpr int L_OUT = 1 - L_IN;
+    extern __shared__ uint64_t xs[];    // [S][SP]
     __shared__ TileShared T;
     tile_setup(P, S, T);
+    // the bits above the window are the same for the whole tile
+    const int64_t top = (xcd_tile(blockIdx.x, gridDim.x) / P.n_col_tiles) / P.n_bot_tiles;
 
-    // ---- load: thread t always serves sub-problem t % S, rows t / S + k * TL
-    {
-        const int s = threadIdx.x % S;
-        const int w0 = threadIdx.x / S;
-        if (T.valid[s]) {
-            const LimbInfo& L = P.limbs[T.limb[s]];
-            const uint64_t q = L.q;
-            uint64_t* xs_s = xs + s * WP;
-            const int es = P.src.kind == 1 ? 4 : 8;
-            if (P.src.kind == 3) {
-                // digit d of x: the reference's unsigned digit loop
-                // (example_impl/g_decomposition_impl.cpp:145-149) in closed form
-                const char* base = (const char*)P.src.ptr + 8 * (T.src_off[s] + T.jbase[s] * P.src.row_stride);
-                const int64_t wstride = (int64_t)8 * (P.src.row_stride << P.b0);
-                const int d = T.digit[s];
-                const int wb = P.gd_w;
-                const uint64_t mask = wb ? (~0ull >> (64 - wb)) : 0;
-                const int shift = wb * d;
-                const int rest = wb * P.gd_D;
-                unsigned long long over = 0;
-                for (int w = w0; w < W; w += TL) {
-                    const int64_t x = *(const int64_t*)(base + (int64_t)w * wstride);
-                    const uint64_t u = (uint64_t)x;
-                    uint64_t v = (wb && shift < 64) ? ((u >> shift) & mask) : 0;
-                    if (d == 0) over += wb ? (rest < 64 && (u >> rest) != 0) : (x > 0);
-                    if (P.reduce_in && v >= q) v = residue(limb_modinfo(L), (int64_t)v);
-                    xs_s[r8_pad(w)] = v;
-                }
-                if (over) atomicAdd(P.gd_status, over);
-            } else if (!P.src_map) {
-                const char* base = (const char*)P.src.ptr + es * (T.src_off[s] + T.jbase[s] * P.src.row_stride);
-                const int64_t wstride = (int64_t)es * (P.src.row_stride << P.b0);
-#pragma unroll 4
-                for (int w = w0; w < W; w += TL) {
-                    const char* ptr = base + (int64_t)w * wstride;
-                    int64_t x = es == 4 ? (int64_t) * (const int32_t*)ptr : *(const int64_t*)ptr;
-                    uint64_t v = (uint64_t)x;
-                    if (P.reduce_in && (x < 0 || v >= q)) v = residue(limb_modinfo(L), x);
-                    xs_s[r8_pad(w)] = v;
-                }
+    const int s = threadIdx.x % S, lane = threadIdx.x / S;
+    const bool valid = T.valid[s];
+    const int limb = T.limb[s];
+    const LimbInfo& L = P.limbs[limb];
+    const uint64_t q = L.q;
+    const int mode = limb_mode(L);
+    const int64_t jbase = T.jbase[s];
+    uint64_t x[16];
+
+    // ---- load straight into the first layout
+    if (valid) {
+        const int es = P.src.kind == 1 ? 4 : 8;
+        const int64_t wstride = (int64_t)es * (P.src.row_stride << P.b0);  // bytes per window row
+        const char* base = (const char*)P.src.ptr + es * (T.src_off[s] + jbase * P.src.row_stride) +
+                           (int64_t)r16_wl<H>(L_IN, lane) * wstride;
+        if (P.src.kind == 3) {
+            // digit d of x: the reference's unsigned digit loop
+            // (example_impl/g_decomposition_impl.cpp:145-149) in closed form
+            const int d = T.digit[s];
+            const int wb = P.gd_w;
+            const uint64_t mask = wb ? (~0ull >> (64 - wb)) : 0;
+            const int shift = wb * d;
+            const int rest = wb * P.gd_D;
+            unsigned long long over = 0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t xv = *(const int64_t*)(base + r16_wc<H>(L_IN, e) * wstride);
+                const uint64_t u = (uint64_t)xv;
+                uint64_t v = (wb && shift < 64) ? ((u >> shift) & mask) : 0;
+                if (d == 0) over += wb ? (rest < 64 && (u >> rest) != 0) : (xv > 0);
+                if (P.reduce_in && v >= q) v = residue(limb_modinfo(L), (int64_t)v);
+                x[e] = v;
+            }
+            if (over) atomicAdd(P.gd_status, over);
+        } else if (!P.src_map) {
+            int64_t xv[16];
+            if (es == 8) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(base + r16_wc<H>(L_IN, e) * wstride);
             } else {
-                for (int w = w0; w < W; w += TL) {
-                    int64_t j = T.jbase[s] + ((int64_t)w << P.b0);
-                    int32_t r = P.src_map[j];
-                    int64_t x = r >= 0 ? load_any(P.src, T.src_off[s] + (int64_t)r * P.src.row_stride)
-                                       : load_any(P.alt, T.alt_off[s] + j * P.alt.row_stride);
-                    uint64_t v = (uint64_t)x;
-                    if (P.reduce_in && (x < 0 || v >= q)) v = residue(limb_modinfo(L), x);
-                    xs_s[r8_pad(w)] = v;
-                }
+#pragma unroll
+                for (int e = 0; e < 16; ++e) xv[e] = *(const int32_t*)(base + r16_wc<H>(L_IN, e) * wstride);
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                uint64_t v = (uint64_t)xv[e];
+                if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
+                x[e] = v;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t j = jbase + ((int64_t)r16_w<H>(L_IN, e, lane) << P.b0);
+                const int32_t r = P.src_map[j];
+                const int64_t xv = r >= 0 ? load_any(P.src, T.src_off[s] + (int64_t)r * P.src.row_stride)
+                                          : load_any(P.alt, T.alt_off[s] + j * P.alt.row_stride);
+                uint64_t v = (uint64_t)xv;
+                if (P.reduce_in && (xv < 0 || v >= q)) v = residue(limb_modinfo(L), xv);
+                x[e] = v;
             }
         }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e] = 0;
     }
+
+    const ulonglong2* twrow = (const ulonglong2*)(P.tw + 2 * (int64_t)limb * P.m);
+    // ---- first stages, exchange, remaining stages
+    if (valid) r16_stages_any<H, INV, L_IN>(x, lane, twrow, top, P.logm, P.stage0, q, mode);
+    uint64_t* xs_s = xs + s * SP;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xs_s[r16_w<H>(L_IN, e, lane)] = x[e];
     __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = xs_s[r16_w<H>(L_OUT, e, lane)];
+    if (!valid) return;
+    r16_stages_any<H, INV, L_OUT>(x, lane, twrow, top, P.logm, P.stage0, q, mode);
 
-    // ---- the window's stages
-    {
-        const int s = threadIdx.x / TL, lane = threadIdx.x % TL;
-        const LimbInfo& L = P.limbs[T.limb[s]];
-        TwRow R;
-        R.tw = (const ulonglong2*)(P.tw + 2 * (int64_t)T.limb[s] * P.m);
-        R.top = T.jbase[s] >> (P.b0 + H);
-        R.logm = P.logm;
-        R.g0 = P.stage0;
-        // invalid sub-problems (past the last column) compute on stale LDS
-        // values and are never stored
-        r8_all_rounds<H, INV>(xs + s * WP, lane, R, L.q, limb_mode(L));
+    // ---- final values: m^-1 scaling, or full reduction of lazy residues
+    if (P.scale_out) {
+        if (mode == MODE_S32) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                x[e] = shoup_mul32((uint32_t)x[e], (uint32_t)L.minv, (uint32_t)L.minv_p, (uint32_t)q);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = shoup_mul(x[e], L.minv, L.minv_p, q);
+        }
+    } else if (!P.lazy_out && mode == MODE_L64) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], q);
     }
 
-    // ---- store
-    {
-        const int s = threadIdx.x % S;
-        const int w0 = threadIdx.x / S;
-        if (!T.valid[s]) return;
-        const LimbInfo& L = P.limbs[T.limb[s]];
-        const uint64_t q = L.q;
-        const int mode = limb_mode(L);
-        const uint64_t* xs_s = xs + s * WP;
-        const int es = P.dst.kind == 1 ? 4 : 8;
-        const bool full = !P.lazy_out;
-        const int64_t colbase = T.dst_off[s];
-        for (int w = w0; w < W; w += TL) {
-            uint64_t v = xs_s[r8_pad(w)];
-            if (P.scale_out) {
-                if (mode == MODE_S32) v = shoup_mul32((uint32_t)v, (uint32_t)L.minv, (uint32_t)L.minv_p, (uint32_t)q);
-                else v = shoup_mul(v, L.minv, L.minv_p, q);
-            } else if (full && mode == MODE_L64) {
-                v = lazy_reduce(v, q);
-            }
-            int64_t j = T.jbase[s] + ((int64_t)w << P.b0);
-            int64_t row = P.dst_map_limb ? (int64_t)P.dst_map_limb[(int64_t)T.limb[s] * P.m + j]
-                                         : (P.dst_map ? (int64_t)P.dst_map[j] : j);
-            char* ptr = (char*)P.dst.ptr + es * (colbase + row * P.dst.row_stride);
-            if (es == 4) *(int32_t*)ptr = (int32_t)v;
-            else *(uint64_t*)ptr = v;
+    // ---- store from registers
+    const int es = P.dst.kind == 1 ? 4 : 8;
+    const int64_t colbase = T.dst_off[s];
+    if (!P.dst_map && !P.dst_map_limb) {
+        const int64_t wstride = (int64_t)es * (P.dst.row_stride << P.b0);
+        char* base = (char*)P.dst.ptr + es * (colbase + jbase * P.dst.row_stride) +
+                     (int64_t)r16_wl<H>(L_OUT, lane) * wstride;
+        if (es == 8) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) *(uint64_t*)(base + r16_wc<H>(L_OUT, e) * wstride) = x[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) *(int32_t*)(base + r16_wc<H>(L_OUT, e) * wstride) = (int32_t)x[e];
         }
+        return;
+    }
+    const int32_t* map = P.dst_map_limb ? P.dst_map_limb + (int64_t)limb * P.m : P.dst_map;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int64_t j = jbase + ((int64_t)r16_w<H>(L_OUT, e, lane) << P.b0);
+        char* ptr = (char*)P.dst.ptr + es * (colbase + (int64_t)map[j] * P.dst.row_stride);
+        if (es == 4) *(int32_t*)ptr = (int32_t)x[e];
+        else *(uint64_t*)ptr = x[e];
     }
 }
 
@@ -709,7 +729,7 @@ ColAddr tmp_addr(void* ptr, int64_t ncols) {
     return c;
 }
 
-bool r8_window(int H) {
+bool r16_window(int H) {
     static const bool generic_only = [] {
         const char* e = getenv("HEAL_AMD_NTT_GENERIC");
         return e && e[0] == '1';
@@ -717,60 +737,66 @@ bool r8_window(int H) {
     return !generic_only && H >= 6 && H <= 8;
 }
 
+constexpr int kR16Tile = 32;  // sub-problems per workgroup of the radix-16 pass
+
 // Launches one pass.  P.S_cols / P.S_bot give the tile (sub-problems per
-// workgroup); for register-radix-8 windows each sub-problem takes 2^(H-3) lanes.
+// workgroup, see configure_tile); a radix-16 sub-problem takes 2^(H-4) lanes.
 void launch_pass(PassArgs& P, hipStream_t st) {
     const int W = 1 << P.H;
-    int S = P.S_cols * P.S_bot;
-    int64_t ntop = int64_t(1) << (P.logm - P.b0 - P.H);
+    const int S = P.S_cols * P.S_bot;
+    const int64_t ntop = int64_t(1) << (P.logm - P.b0 - P.H);
     P.n_bot_tiles = (P.bot_count + P.S_bot - 1) / P.S_bot;
     P.n_col_tiles = (P.ncols + P.S_cols - 1) / P.S_cols;
-    int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
+    const int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
     if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
-    if (r8_window(P.H)) {
-        const int TL = W / 8;
-        const int threads = S * TL;
-        if (threads != 256 && threads != 512) raise(HX_ERR_RUNTIME, "internal: radix-8 tile size");
-        size_t lds = (size_t)S * (W + W / 32) * 8;
-        dim3 grid((unsigned)blocks), block((unsigned)threads);
-#define HX_R8(HH, TT)                                                                              \
-    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r8<HH, true, TT>), grid, block, lds, st, P);     \
-    else hipLaunchKernelGGL((k_ntt_pass_r8<HH, false, TT>), grid, block, lds, st, P);
-        if (threads == 512) {
-            if (P.H == 8) {
-                HX_R8(8, 512)
-            } else if (P.H == 7) {
-                HX_R8(7, 512)
-            } else {
-                HX_R8(6, 512)
-            }
+    if (r16_window(P.H)) {
+        if (S != kR16Tile) raise(HX_ERR_RUNTIME, "internal: radix-16 tile size");
+        const size_t lds = (size_t)kR16Tile * (W + 1) * 8;
+        const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
+#define HX_R16(HH)                                                                            \
+    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true>), grid, block, lds, st, P);   \
+    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false>), grid, block, lds, st, P);
+        if (P.H == 8) {
+            HX_R16(8)
+        } else if (P.H == 7) {
+            HX_R16(7)
         } else {
-            if (P.H == 8) {
-                HX_R8(8, 256)
-            } else if (P.H == 7) {
-                HX_R8(7, 256)
-            } else {
-                HX_R8(6, 256)
-            }
+            HX_R16(6)
         }
-#undef HX_R8
+#undef HX_R16
     } else {
-        size_t lds = (size_t)S * (W + 1) * 8;
+        const size_t lds = (size_t)S * (W + 1) * 8;
         hipLaunchKernelGGL(k_ntt_pass, dim3((unsigned)blocks), dim3(kThreads), lds, st, P);
     }
     check_hip(hipGetLastError(), "ntt pass launch");
 }
 
-// Sub-problems per workgroup for a window of H bits.
-int tile_subproblems(int H, int npass, bool many_cols) {
-    const int W = 1 << H;
-    if (r8_window(H)) {
-        const int TL = W / 8;
-        return (many_cols ? 512 : 256) / TL;
+// Tile shape of a pass: col_tiles puts adjacent columns of one row in a tile
+// (layouts with the limb axis innermost), otherwise adjacent rows of one
+// column (limb-major layouts).  P.H, P.ncols and P.bot_count must be set.
+void configure_tile(PassArgs& P, bool col_tiles, int npass) {
+    if (r16_window(P.H)) {
+        if (col_tiles) {
+            int sc = 1;
+            while (sc < kR16Tile && sc < P.ncols) sc <<= 1;
+            P.S_cols = sc;
+            P.S_bot = kR16Tile / sc;
+        } else {
+            P.S_cols = 1;
+            P.S_bot = kR16Tile;
+        }
+        return;
     }
+    const int W = 1 << P.H;
     int S = W >= 256 ? 8 : std::max(1, std::min(64, 2048 / W));
     if (npass == 1 && W > 256) S = std::max(1, 4096 / W);
-    return S;
+    if (!col_tiles) {
+        P.S_cols = 1;
+        P.S_bot = (int)std::max<int64_t>(1, std::min<int64_t>(S, P.bot_count));
+    } else {
+        P.S_cols = S;
+        P.S_bot = 1;
+    }
 }
 
 }  // namespace
