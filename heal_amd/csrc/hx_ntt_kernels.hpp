@@ -412,14 +412,14 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
         const uint64_t q2 = q << 1;
         if (!INV) {  // x0, x1 in [0, 4q) -> [0, 4q)
             uint64_t u = x0 >= q2 ? x0 - q2 : x0;
-            uint64_t t = shoup_lazy(x1, w, wp, q);  // [0, 2q)
+            uint64_t t = shoup_lazy_nq(x1, w, wp, (uint64_t)0 - q);  // [0, 2q)
             x0 = u + t;
             x1 = u - t + q2;
         } else {  // x0, x1 in [0, 2q) -> [0, 2q)
             uint64_t a = x0 + x1;
             uint64_t d = x0 - x1 + q2;
             x0 = a >= q2 ? a - q2 : a;
-            x1 = shoup_lazy(d, w, wp, q);
+            x1 = shoup_lazy_nq(d, w, wp, (uint64_t)0 - q);
         }
     } else {
         if (!INV) {
@@ -470,30 +470,69 @@ __device__ __forceinline__ int r16_wB(int e, int lane) {
 // The stages of one layout on the 16 registers of a lane.  The twiddle index
 // of a butterfly is ubase(stage) + (w >> (b + 1)) for window bit b; in layout
 // A that is ubase + (e >> (b - nb + 1)), in layout B ubase + lane part + a
-// constant, so each stage needs one address and immediate offsets.
-template <int H, bool INV, int MODE, int LAYOUT>
-__device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ulonglong2* twrow, int64_t top,
-                                           int logm, int g0, uint64_t q) {
-    constexpr int nb = H - 4;
-    constexpr int nst = LAYOUT == 0 ? 4 : nb;
-#pragma unroll
-    for (int k = 0; k < nst; ++k) {
-        // forward: high window bits first; inverse: low bits first
-        const int b = LAYOUT == 0 ? (INV ? nb + k : H - 1 - k) : (INV ? k : nb - 1 - k);
-        const int eb = LAYOUT == 0 ? b - nb : b;  // the bit of e that b is
-        const int st = INV ? b : H - 1 - b;       // stage within the window
-        const int g = g0 + st;                    // global stage
+// constant, so each stage needs one address and immediate offsets.  The
+// twiddles of stage k + 1 are requested before the butterflies of stage k run,
+// which hides their load latency behind one stage of arithmetic.
+template <int H, bool INV, int LAYOUT>
+struct R16Stage {
+    static constexpr int nb = H - 4;
+    __device__ static constexpr int bit(int k) {  // forward: high bits first; inverse: low bits first
+        return LAYOUT == 0 ? (INV ? nb + k : H - 1 - k) : (INV ? k : nb - 1 - k);
+    }
+    __device__ static constexpr int ebit(int k) { return LAYOUT == 0 ? bit(k) - nb : bit(k); }
+    // twiddle offset of the butterfly whose lower register is e
+    __device__ static constexpr int toff(int k, int e) {
+        return LAYOUT == 0 ? (e >> (bit(k) - nb + 1))
+                           : (((e >> nb) << (2 * nb - bit(k) - 1)) | ((e & ((1 << nb) - 1)) >> (bit(k) + 1)));
+    }
+    // the 8 twiddles of stage k, in butterfly order (equal addresses are loaded once)
+    __device__ static __forceinline__ void load(ulonglong2 (&tw)[8], int k, int lane, const ulonglong2* twrow,
+                                                int64_t top, int logm, int g0) {
+        const int b = bit(k);
+        const int st = INV ? b : H - 1 - b;  // stage within the window
+        const int g = g0 + st;               // global stage
         const int64_t ubase = INV ? ((int64_t(1) << (logm - 1 - g)) + (top << (H - 1 - b)))
                                   : ((int64_t(1) << g) + (top << st));
         const ulonglong2* tp = twrow + ubase + (LAYOUT == 1 ? ((int64_t)lane << (nb - b - 1)) : 0);
+        int i = 0;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (e & (1 << ebit(k))) continue;
+            tw[i++] = tp[toff(k, e)];
+        }
+    }
+};
+
+template <int H, bool INV, int MODE, int LAYOUT>
+__device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ulonglong2* twrow, int64_t top,
+                                           int logm, int g0, uint64_t q) {
+    using St = R16Stage<H, INV, LAYOUT>;
+    constexpr int nst = LAYOUT == 0 ? 4 : H - 4;
+    ulonglong2 cur[8], nxt[8];
+    __builtin_amdgcn_sched_barrier(0);  // the first twiddles are not hoisted into the caller's loads
+    St::load(cur, 0, lane, twrow, top, logm, g0);
+#pragma unroll
+    for (int k = 0; k < nst; ++k) {
+        // the inverse's first stage already holds 8 twiddles: its successor is
+        // loaded after it (keeps the kernel at 4 waves per SIMD)
+        const bool prefetch = k + 1 < nst && !(INV && k == 0);
+        if (prefetch) {
+            St::load(nxt, k + 1, lane, twrow, top, logm, g0);
+            __builtin_amdgcn_sched_barrier(0);  // keep the requests ahead of this stage's arithmetic
+        }
+        const int eb = St::ebit(k);
+        int i = 0;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             if (e & (1 << eb)) continue;
-            const int e2 = e | (1 << eb);
-            const int c = LAYOUT == 0 ? (e >> (b - nb + 1))
-                                      : (((e >> nb) << (2 * nb - b - 1)) | ((e & ((1 << nb) - 1)) >> (b + 1)));
-            const ulonglong2 tw = tp[c];
-            butterfly<MODE, INV>(x[e], x[e2], tw.x, tw.y, q);
+            butterfly<MODE, INV>(x[e], x[e | (1 << eb)], cur[i].x, cur[i].y, q);
+            ++i;
+        }
+        if (prefetch) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
+        } else if (k + 1 < nst) {
+            St::load(cur, k + 1, lane, twrow, top, logm, g0);
         }
     }
 }
