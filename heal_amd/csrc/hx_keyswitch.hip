@@ -101,7 +101,7 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
             const uint64_t* src = A.src + (row0 + lane) * ncols1 + (s % D) + (int64_t)D * (t + A.k * b);
 #pragma unroll
             for (int e = 0; e < 16; ++e) x[e] = src[(int64_t)(e * TL) * ncols1];
-            r16_stages_any<H, false, 0>(x, lane, twrow, hi, A.logm, A.hhi, L.q, mode);
+            r16_stages_any<H, false, 0>(x, lane, twrow, hi, A.logm, A.hhi, L, mode);
         }
         uint64_t* xs = lds + s * SP;
         if (act) {
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
         if (valid) {
 #pragma unroll
             for (int e = 0; e < 16; ++e) x[e] = xs[r16_wB<H>(e, lane)];
-            r16_stages_any<H, false, 1>(x, lane, twrow, hi, A.logm, A.hhi, L.q, mode);
+            r16_stages_any<H, false, 1>(x, lane, twrow, hi, A.logm, A.hhi, L, mode);
             if (mode == MODE_L64) {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], L.q);
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
 
     // ---- inverse stages over the low bits (global stages 0 .. hlo-1)
     const ulonglong2* twrow = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
-    if (valid) r16_stages_any<H, true, 1>(x, lane, twrow, hi, A.logm, 0, q, mode);
+    if (valid) r16_stages_any<H, true, 1>(x, lane, twrow, hi, A.logm, 0, L, mode);
     __syncthreads();  // every read of the forward values is done
     uint64_t* xs = lds + s * SP;
     if (act) {
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
     if (!valid) return;
 #pragma unroll
     for (int e = 0; e < 16; ++e) x[e] = xs[r16_wA<H>(e, lane)];
-    r16_stages_any<H, true, 0>(x, lane, twrow, hi, A.logm, 0, q, mode);
+    r16_stages_any<H, true, 0>(x, lane, twrow, hi, A.logm, 0, L, mode);
 
     // ---- store (residues stay lazy for the next radix-16 pass)
     uint64_t* dst = A.dst + (row0 + lane) * ncols2 + c + 2 * (t + A.k * b);

@@ -92,7 +92,7 @@ hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm,
                                plan->limbs, flags);
             if (m > 0)
                 hipLaunchKernelGGL(k_plan_twiddles, dim3(grid_for(k * m, 256, 8192)), dim3(256), 0, st, plan_src(tw), k,
-                                   m, plan->limbs, plan->tw);
+                                   m, plan->limbs, inverse, plan->tw);
         }
         if (m > 0) {
             unsigned g = grid_for(m, 256, 4096);

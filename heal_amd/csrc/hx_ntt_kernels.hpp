@@ -19,7 +19,8 @@ struct LimbInfo {
     uint32_t small;  // 1: q < 2^31, butterflies in 32-bit arithmetic
     uint64_t minv;   // m^-1 mod q (inverse plans)
     uint64_t minv_p; // its Shoup companion (32- or 64-bit)
-    uint64_t pad[2];
+    uint64_t wm;     // inv_twiddles[t, 1] * m^-1 mod q: the last inverse stage with the scaling folded in
+    uint64_t wm_p;   // its Shoup companion
 };
 
 struct hx_ntt_plan {
@@ -111,16 +112,22 @@ __global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, L
     limbs[t] = L;
 }
 
-__global__ void k_plan_twiddles(PlanSrc tw, int64_t k, int64_t m, const LimbInfo* limbs, uint64_t* out) {
+__global__ void k_plan_twiddles(PlanSrc tw, int64_t k, int64_t m, LimbInfo* limbs, int inverse, uint64_t* out) {
     int64_t n = k * m;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t t = e / m, i = e - t * m;
-        LimbInfo L = limbs[t];
+        const LimbInfo& L = limbs[t];  // one thread per limb writes wm, wm_p; the others read q, d, v, s, small
         ModInfo mi = limb_modinfo(L);
         uint64_t w = residue(mi, ld_elem(tw.ptr, t * tw.s0 + i * tw.s1, tw.elem32));
         uint64_t wp = L.small ? divq128(mi, 0, w << 32) : shoup_companion(mi, w);
         out[2 * e] = w;
         out[2 * e + 1] = wp;
+        if (inverse && i == 1) {
+            // the last Gentleman-Sande stage uses inv_twiddles[t, 1] only
+            const uint64_t wm = mulmod(mi, w, L.minv);
+            limbs[t].wm = wm;
+            limbs[t].wm_p = L.small ? divq128(mi, 0, wm << 32) : shoup_companion(mi, wm);
+        }
     }
 }
 
@@ -503,11 +510,31 @@ struct R16Stage {
     }
 };
 
+// Last inverse stage with the m^-1 scaling folded in: (u + v) * m^-1 and
+// (u - v) * (inv_twiddles[t, 1] * m^-1), fully reduced.
+template <int MODE>
+__device__ __forceinline__ void butterfly_last_scaled(uint64_t& x0, uint64_t& x1, const LimbInfo& L) {
+    const uint64_t q = L.q;
+    if (MODE == MODE_S32) {
+        const uint32_t q32 = (uint32_t)q, u = (uint32_t)x0, v = (uint32_t)x1;
+        x0 = shoup_mul32(u + v, (uint32_t)L.minv, (uint32_t)L.minv_p, q32);
+        x1 = shoup_mul32(u - v + q32, (uint32_t)L.wm, (uint32_t)L.wm_p, q32);
+    } else {
+        const uint64_t lift = MODE == MODE_L64 ? q << 1 : q;  // inputs below 2q (lazy) or below q
+        const uint64_t u = x0, v = x1;
+        x0 = shoup_mul(u + v, L.minv, L.minv_p, q);
+        x1 = shoup_mul(u - v + lift, L.wm, L.wm_p, q);
+    }
+}
+
+// fold: the last stage is the transform's last inverse stage, scaled by m^-1
+// (a uniform flag; only the last stage has two variants)
 template <int H, bool INV, int MODE, int LAYOUT>
 __device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ulonglong2* twrow, int64_t top,
-                                           int logm, int g0, uint64_t q) {
+                                           int logm, int g0, const LimbInfo& L, bool fold) {
     using St = R16Stage<H, INV, LAYOUT>;
     constexpr int nst = LAYOUT == 0 ? 4 : H - 4;
+    const uint64_t q = L.q;
     ulonglong2 cur[8], nxt[8];
     __builtin_amdgcn_sched_barrier(0);  // the first twiddles are not hoisted into the caller's loads
     St::load(cur, 0, lane, twrow, top, logm, g0);
@@ -521,12 +548,20 @@ __device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ul
             __builtin_amdgcn_sched_barrier(0);  // keep the requests ahead of this stage's arithmetic
         }
         const int eb = St::ebit(k);
-        int i = 0;
+        if (INV && LAYOUT == 0 && k == nst - 1 && fold) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            if (e & (1 << eb)) continue;
-            butterfly<MODE, INV>(x[e], x[e | (1 << eb)], cur[i].x, cur[i].y, q);
-            ++i;
+            for (int e = 0; e < 16; ++e) {
+                if (e & (1 << eb)) continue;
+                butterfly_last_scaled<MODE>(x[e], x[e | (1 << eb)], L);
+            }
+        } else {
+            int i = 0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                if (e & (1 << eb)) continue;
+                butterfly<MODE, INV>(x[e], x[e | (1 << eb)], cur[i].x, cur[i].y, q);
+                ++i;
+            }
         }
         if (prefetch) {
 #pragma unroll
@@ -539,10 +574,10 @@ __device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ul
 
 template <int H, bool INV, int LAYOUT>
 __device__ __forceinline__ void r16_stages_any(uint64_t (&x)[16], int lane, const ulonglong2* twrow, int64_t top,
-                                               int logm, int g0, uint64_t q, int mode) {
-    if (mode == MODE_L64) r16_stages<H, INV, MODE_L64, LAYOUT>(x, lane, twrow, top, logm, g0, q);
-    else if (mode == MODE_S32) r16_stages<H, INV, MODE_S32, LAYOUT>(x, lane, twrow, top, logm, g0, q);
-    else r16_stages<H, INV, MODE_F64, LAYOUT>(x, lane, twrow, top, logm, g0, q);
+                                               int logm, int g0, const LimbInfo& L, int mode, bool fold = false) {
+    if (mode == MODE_L64) r16_stages<H, INV, MODE_L64, LAYOUT>(x, lane, twrow, top, logm, g0, L, fold);
+    else if (mode == MODE_S32) r16_stages<H, INV, MODE_S32, LAYOUT>(x, lane, twrow, top, logm, g0, L, fold);
+    else r16_stages<H, INV, MODE_F64, LAYOUT>(x, lane, twrow, top, logm, g0, L, fold);
 }
 
 template <int H>
@@ -626,6 +661,30 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
                 if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
                 x[e] = v;
             }
+        } else if (P.b0 == 0 && !P.src_map_alt && INV && es == 8) {
+            // inverse row gather of the first pass: the 16 map entries of a lane
+            // form runs of 2^(H-4) >= 4 consecutive rows, read as int4
+            int32_t r[16];
+            const int32_t* mp = P.src_map + jbase + r16_wl<H>(L_IN, lane);
+#pragma unroll
+            for (int e = 0; e < 16; e += 4) {
+                const int4 v4 = *(const int4*)(mp + r16_wc<H>(L_IN, e));
+                r[e] = v4.x;
+                r[e + 1] = v4.y;
+                r[e + 2] = v4.z;
+                r[e + 3] = v4.w;
+            }
+            const char* colp = (const char*)P.src.ptr + 8 * T.src_off[s];
+            const int64_t rs = 8 * P.src.row_stride;
+            int64_t xv[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(colp + (int64_t)r[e] * rs);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                uint64_t v = (uint64_t)xv[e];
+                if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
+                x[e] = v;
+            }
         } else {
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
@@ -645,7 +704,7 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
 
     const ulonglong2* twrow = (const ulonglong2*)(P.tw + 2 * (int64_t)limb * P.m);
     // ---- first stages, exchange, remaining stages
-    if (valid) r16_stages_any<H, INV, L_IN>(x, lane, twrow, top, P.logm, P.stage0, q, mode);
+    if (valid) r16_stages_any<H, INV, L_IN>(x, lane, twrow, top, P.logm, P.stage0, L, mode);
     uint64_t* xs_s = xs + s * SP;
 #pragma unroll
     for (int e = 0; e < 16; ++e) xs_s[r16_w<H>(L_IN, e, lane)] = x[e];
@@ -653,19 +712,12 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
 #pragma unroll
     for (int e = 0; e < 16; ++e) x[e] = xs_s[r16_w<H>(L_OUT, e, lane)];
     if (!valid) return;
-    r16_stages_any<H, INV, L_OUT>(x, lane, twrow, top, P.logm, P.stage0, q, mode);
+    // the last pass of an inverse transform ends with the global stage logm - 1
+    // (window bit H - 1, the last stage of layout A), where m^-1 is folded in
+    r16_stages_any<H, INV, L_OUT>(x, lane, twrow, top, P.logm, P.stage0, L, mode, INV && P.scale_out);
 
-    // ---- final values: m^-1 scaling, or full reduction of lazy residues
-    if (P.scale_out) {
-        if (mode == MODE_S32) {
-#pragma unroll
-            for (int e = 0; e < 16; ++e)
-                x[e] = shoup_mul32((uint32_t)x[e], (uint32_t)L.minv, (uint32_t)L.minv_p, (uint32_t)q);
-        } else {
-#pragma unroll
-            for (int e = 0; e < 16; ++e) x[e] = shoup_mul(x[e], L.minv, L.minv_p, q);
-        }
-    } else if (!P.lazy_out && mode == MODE_L64) {
+    // ---- full reduction of lazy residues (scaled outputs are already reduced)
+    if (!P.scale_out && !P.lazy_out && mode == MODE_L64) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], q);
     }
