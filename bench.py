@@ -21,6 +21,10 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the forward pass kernel on the bench workload, keyed by
+# (N, L, B): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
+# counter's reading for a 256 MiB copy (profiles/ntt_pass_hbm_traffic.md)
+NTT_PASS_TRAFFIC = {(65536, 24, 8): 633131302}
 
 
 def parse():
@@ -160,17 +164,22 @@ def main():
     ks_plain = args.steps / t_plain
     fused_gbs = batched.fused_bytes() * args.steps / t_graph / 1e9
 
-    # NTT pass kernel: forward digit NTTs of the batch (B*dnum*L limbs of N points);
-    # each of the 2 launches reads and writes every limb once.
+    # NTT roofline: the forward digit NTTs of the batch (B*dnum*L limbs of N
+    # points) run as two launches of the radix-16 pass kernel (one when N <= 2048).
+    # Each launch runs half of every limb's stages, so its algorithmic share is
+    # half of the transform's 2*N*8 bytes per limb: the re-read of the
+    # intermediate between the launches is not credited.
     limbs_fwd = B * batched.dnum * L
-    bytes_per_launch = limbs_fwd * 2 * N * 8
+    ntt_bytes = limbs_fwd * 2 * N * 8
     reps = 10
     dig_view = batched.digits.permute(0, 1, 3, 2)
     t_ntt = timed(lambda: batched.fwd(dig_view, batched.dntt), reps, torch)
-    n_launch = 2 if N > 4096 else 1
+    n_launch = 2 if N > 2048 else 1
+    bytes_per_launch = ntt_bytes // n_launch
     avg_launch = t_ntt / (reps * n_launch)
     achieved = bytes_per_launch / avg_launch / 1e9
-    ntt_limb_gbs = limbs_fwd * 2 * N * 8 * reps / t_ntt / 1e9
+    ntt_limb_gbs = ntt_bytes * reps / t_ntt / 1e9
+    traffic = NTT_PASS_TRAFFIC.get((N, L, B))
     limbs_inv = B * 2 * L
     t_intt = timed(lambda: batched.inv(batched.acc.view(B * 2, N, 1, L), batched.coef), reps, torch)
     intt_limb_gbs = limbs_inv * 2 * N * 8 * reps / t_intt / 1e9
@@ -200,10 +209,13 @@ def main():
                    "keyswitch_algorithmic_bytes": batched.algorithmic_bytes() // B,
                    "keyswitch_chain_bytes": batched.chain_bytes() // B,
                    "keyswitch_fused_bytes": batched.fused_bytes() // B},
-        "roofline": {"bound": "hbm", "kernel": "k_ntt_pass (forward, 2 launches per NTT)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_ntt_pass_r16<8,false>: forward NTT of {limbs_fwd} limbs of N={N}, "
+                               f"{n_launch} launches per NTT",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4),
+                     "launch_read_write_GBps": round(2 * achieved, 1)},
         "cpu_baseline": None,
     }
     try:
