@@ -41,6 +41,7 @@ struct DeviceTensor {
     std::vector<int64_t> dims;
     std::vector<int64_t> strides;  // in elements; 0 along broadcast axes
     std::shared_ptr<lattica_hw_api::detail::Buffer> buffer;
+    int64_t offset = 0;  // elements from the buffer start (views made by get_slice)
 
     // reference semantics (example_impl/device_memory_impl.cpp:78-120): the new
     // shape must have as many elements as the dims with non-zero stride; the
@@ -50,7 +51,7 @@ struct DeviceTensor {
     void print_metadata() const;
     // C-contiguous, ignoring size-1 dims
     bool is_contiguous() const;
-    T* data_ptr() const { return buffer ? static_cast<T*>(buffer->ptr) : nullptr; }
+    T* data_ptr() const { return buffer ? static_cast<T*>(buffer->ptr) + offset : nullptr; }
 };
 
 namespace lattica_hw_api {

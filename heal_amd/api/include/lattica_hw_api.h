@@ -86,6 +86,48 @@ template <typename T>
 void permute(const std::shared_ptr<DeviceTensor<T>>& a, const std::shared_ptr<DeviceTensor<T>>& perms,
              std::shared_ptr<DeviceTensor<T>>& result, int64_t elementwise_axis, int64_t perm_axis);
 
+// ---- transcript operations that the reference's runtime names but no reference
+// file defines (python_execution/lattica_heal_runtime/device_interface.py:47-48,
+// 75-107).  Semantics are this library's own (stated per function).
+
+// result = the residue of -a modulo |p| in [0, |p|) (0 where p == 0); p broadcasts as in modmul
+template <typename T>
+void modneg_tt(const std::shared_ptr<DeviceTensor<T>>& a, const std::shared_ptr<DeviceTensor<T>>& p,
+               std::shared_ptr<DeviceTensor<T>>& result);
+template <typename T>
+void modneg_tc(const std::shared_ptr<DeviceTensor<T>>& a, T p_scalar, std::shared_ptr<DeviceTensor<T>>& result);
+// result = axis_modsum(modmul_ttt(a, b, p), p, axis) without the product tensor; a and b
+// broadcast against each other (right-aligned) and axis counts in the broadcast shape
+template <typename T>
+void modmul_axis_sum(const std::shared_ptr<DeviceTensor<T>>& a, const std::shared_ptr<DeviceTensor<T>>& b,
+                     const std::shared_ptr<DeviceTensor<T>>& p, std::shared_ptr<DeviceTensor<T>>& result,
+                     int64_t axis);
+// result = |a| (two's complement wrap at the minimum)
+template <typename T>
+void abs_values(const std::shared_ptr<DeviceTensor<T>>& a, std::shared_ptr<DeviceTensor<T>>& result);
+// every element of the view := value (the buffer is written in place)
+template <typename T>
+void set_const_val(const std::shared_ptr<DeviceTensor<T>>& a, T value);
+// numpy.moveaxis materialised: a new C-contiguous tensor
+template <typename T>
+std::shared_ptr<DeviceTensor<T>> moveaxis(const std::shared_ptr<DeviceTensor<T>>& a, int64_t source,
+                                          int64_t destination);
+// Basic indexing of the leading dims, numpy style: a view sharing a's buffer.
+// A spec with index == true takes element `start` and drops the dim; otherwise
+// it keeps start, start + step, ... below stop (0 <= start <= stop <= dim, step > 0).
+struct SliceSpec {
+    int64_t start = 0, stop = 0, step = 1;
+    bool index = false;
+};
+template <typename T>
+std::shared_ptr<DeviceTensor<T>> get_slice(const std::shared_ptr<DeviceTensor<T>>& a,
+                                           const std::vector<SliceSpec>& spec);
+// a new C-contiguous tensor with `before` zeros ahead of and `after` zeros behind
+// the data along `axis`
+template <typename T>
+std::shared_ptr<DeviceTensor<T>> pad_single_axis(const std::shared_ptr<DeviceTensor<T>>& a, int64_t axis,
+                                                 int64_t before, int64_t after);
+
 // ---- backend controls (no counterpart in the reference)
 // Stream that subsequent ops of the calling thread are enqueued on (a
 // hipStream_t; nullptr = the legacy default stream).

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <new>
 #include <numeric>
+#include <type_traits>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -129,12 +130,13 @@ void sync_and_report() {
 // ---- caches of prepared device tables, keyed by buffer identity and version
 struct TensorKey {
     uint64_t id = 0, version = 0;
+    int64_t offset = 0;
     std::vector<int64_t> dims, strides;
     bool operator<(const TensorKey& o) const {
-        return std::tie(id, version, dims, strides) < std::tie(o.id, o.version, o.dims, o.strides);
+        return std::tie(id, version, offset, dims, strides) < std::tie(o.id, o.version, o.offset, o.dims, o.strides);
     }
     bool operator==(const TensorKey& o) const {
-        return std::tie(id, version, dims, strides) == std::tie(o.id, o.version, o.dims, o.strides);
+        return std::tie(id, version, offset, dims, strides) == std::tie(o.id, o.version, o.offset, o.dims, o.strides);
     }
 };
 
@@ -144,6 +146,7 @@ TensorKey key_of(const std::shared_ptr<DeviceTensor<T>>& t) {
     if (!t) return k;
     k.id = t->buffer ? t->buffer->id : 0;
     k.version = t->buffer ? t->buffer->version : 0;
+    k.offset = t->offset;
     k.dims = t->dims;
     k.strides = t->strides;
     return k;
@@ -399,6 +402,7 @@ std::shared_ptr<DeviceTensor<T>> make_contiguous(const std::shared_ptr<DeviceTen
     hx_view dst = view_of(dst_t);
     check(hx_strided_copy(&src, &dst, t_stream));
     t->buffer = buf;
+    t->offset = 0;
     t->strides = dst_t.strides;
     return t;
 }
@@ -570,6 +574,130 @@ void permute(const TensorPtr<T>& a, const TensorPtr<T>& perms, TensorPtr<T>& res
     bump(result);
 }
 
+// ------------------------------------------------------------------ extensions
+// Transcript operations named by the reference runtime without a reference
+// definition (python_execution/lattica_heal_runtime/device_interface.py).
+
+template <typename T>
+void modneg_tt(const TensorPtr<T>& a, const TensorPtr<T>& p, TensorPtr<T>& result) {
+    require(a, "a");
+    require(p, "p");
+    require(result, "result");
+    hx_view av = HX_V(a), pv = HX_V(p), rv = HX_V(result);
+    check(hx_modneg_tt(&av, &pv, &rv, t_stream));
+    bump(result);
+}
+
+template <typename T>
+void modneg_tc(const TensorPtr<T>& a, T p_scalar, TensorPtr<T>& result) {
+    require(a, "a");
+    require(result, "result");
+    hx_view av = HX_V(a), rv = HX_V(result);
+    check(hx_modneg_tc(&av, (int64_t)p_scalar, &rv, t_stream));
+    bump(result);
+}
+
+template <typename T>
+void modmul_axis_sum(const TensorPtr<T>& a, const TensorPtr<T>& b, const TensorPtr<T>& p, TensorPtr<T>& result,
+                     int64_t axis) {
+    require(a, "a");
+    require(b, "b");
+    require(p, "p");
+    require(result, "result");
+    hx_view av = HX_V(a), bv = HX_V(b), pv = HX_V(p), rv = HX_V(result);
+    check(hx_modmul_axis_sum(&av, &bv, &pv, &rv, axis, t_stream));
+    bump(result);
+}
+
+template <typename T>
+void abs_values(const TensorPtr<T>& a, TensorPtr<T>& result) {
+    require(a, "a");
+    require(result, "result");
+    hx_view av = HX_V(a), rv = HX_V(result);
+    check(hx_abs(&av, &rv, t_stream));
+    bump(result);
+}
+
+template <typename T>
+void set_const_val(const TensorPtr<T>& a, T value) {
+    require(a, "a");
+    hx_view av = HX_V(a);
+    check(hx_set_const(&av, std::is_floating_point<T>::value ? 0 : (int64_t)value, (double)value, t_stream));
+    bump(a);
+}
+
+static int64_t norm_axis(int64_t axis, int64_t nd, const char* what) {
+    if (axis < -nd || axis >= nd) throw std::invalid_argument(std::string(what) + " out of range");
+    return axis < 0 ? axis + nd : axis;
+}
+
+template <typename T>
+TensorPtr<T> moveaxis(const TensorPtr<T>& a, int64_t source, int64_t destination) {
+    require(a, "a");
+    const int64_t nd = (int64_t)a->dims.size();
+    const int64_t s = norm_axis(source, nd, "source axis"), d = norm_axis(destination, nd, "destination axis");
+    std::vector<int64_t> order;
+    for (int64_t i = 0; i < nd; ++i)
+        if (i != s) order.push_back(i);
+    order.insert(order.begin() + d, s);
+    DeviceTensor<T> src = *a;
+    for (int64_t i = 0; i < nd; ++i) {
+        src.dims[i] = a->dims[order[i]];
+        src.strides[i] = a->strides[order[i]];
+    }
+    auto out = allocate_on_hardware<T>(src.dims);
+    hx_view sv = view_of(src), ov = HX_V(out);
+    check(hx_strided_copy(&sv, &ov, t_stream));
+    return out;
+}
+
+template <typename T>
+TensorPtr<T> get_slice(const TensorPtr<T>& a, const std::vector<SliceSpec>& spec) {
+    require(a, "a");
+    if (spec.size() > a->dims.size()) throw std::out_of_range("too many indices for tensor");
+    auto v = std::make_shared<DeviceTensor<T>>(*a);
+    v->dims.clear();
+    v->strides.clear();
+    for (size_t i = 0; i < a->dims.size(); ++i) {
+        const int64_t dim = a->dims[i], st = a->strides[i];
+        if (i >= spec.size()) {
+            v->dims.push_back(dim);
+            v->strides.push_back(st);
+            continue;
+        }
+        const SliceSpec& sp = spec[i];
+        if (sp.index) {
+            if (sp.start < 0 || sp.start >= dim) throw std::out_of_range("index out of range");
+            v->offset += sp.start * st;
+            continue;
+        }
+        if (sp.step <= 0) throw std::invalid_argument("slice step must be positive");
+        if (sp.start < 0 || sp.stop > dim || sp.start > sp.stop) throw std::out_of_range("slice out of range");
+        const int64_t n = (sp.stop - sp.start + sp.step - 1) / sp.step;
+        v->offset += (n ? sp.start * st : 0);
+        v->dims.push_back(n);
+        v->strides.push_back(st * sp.step);
+    }
+    return v;
+}
+
+template <typename T>
+TensorPtr<T> pad_single_axis(const TensorPtr<T>& a, int64_t axis, int64_t before, int64_t after) {
+    require(a, "a");
+    const int64_t nd = (int64_t)a->dims.size();
+    const int64_t ax = norm_axis(axis, nd, "axis");
+    if (before < 0 || after < 0) throw std::invalid_argument("padding must be non-negative");
+    std::vector<int64_t> dims = a->dims;
+    dims[ax] += before + after;
+    auto out = allocate_on_hardware<T>(dims);
+    DeviceTensor<T> inner = *out;  // the sub-view that receives a
+    inner.dims = a->dims;
+    inner.offset = before * out->strides[ax];
+    hx_view sv = HX_V(a), dv = view_of(inner);
+    check(hx_strided_copy(&sv, &dv, t_stream));
+    return out;
+}
+
 // ------------------------------------------------------------------ instantiations
 #define HX_INST_MEM(T)                                                                                          \
     template std::shared_ptr<DeviceTensor<T>> allocate_on_hardware<T>(const std::vector<int64_t>&);            \
@@ -578,7 +706,11 @@ void permute(const TensorPtr<T>& a, const TensorPtr<T>& perms, TensorPtr<T>& res
     template std::shared_ptr<DeviceTensor<T>> expand<T>(const std::shared_ptr<DeviceTensor<T>>&, int64_t, int64_t); \
     template std::shared_ptr<DeviceTensor<T>> squeeze<T>(const std::shared_ptr<DeviceTensor<T>>&, int64_t);   \
     template std::shared_ptr<DeviceTensor<T>> unsqueeze<T>(const std::shared_ptr<DeviceTensor<T>>&, int64_t); \
-    template std::shared_ptr<DeviceTensor<T>> make_contiguous<T>(const std::shared_ptr<DeviceTensor<T>>&);
+    template std::shared_ptr<DeviceTensor<T>> make_contiguous<T>(const std::shared_ptr<DeviceTensor<T>>&);    \
+    template void set_const_val<T>(const TensorPtr<T>&, T);                                                   \
+    template TensorPtr<T> moveaxis<T>(const TensorPtr<T>&, int64_t, int64_t);                                 \
+    template TensorPtr<T> get_slice<T>(const TensorPtr<T>&, const std::vector<SliceSpec>&);                  \
+    template TensorPtr<T> pad_single_axis<T>(const TensorPtr<T>&, int64_t, int64_t, int64_t);
 
 #define HX_INST_OPS(T)                                                                                     \
     template void modmul_ttt<T>(const TensorPtr<T>&, const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&); \
@@ -598,7 +730,12 @@ void permute(const TensorPtr<T>& a, const TensorPtr<T>& perms, TensorPtr<T>& res
                          const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&);                         \
     template void intt<T>(const TensorPtr<T>&, const TensorPtr<T>&, const TensorPtr<T>&, const TensorPtr<T>&, \
                           const TensorPtr<T>&, const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&);   \
-    template void permute<T>(const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&, int64_t, int64_t);
+    template void permute<T>(const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&, int64_t, int64_t);    \
+    template void modneg_tt<T>(const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&);                   \
+    template void modneg_tc<T>(const TensorPtr<T>&, T, TensorPtr<T>&);                                     \
+    template void modmul_axis_sum<T>(const TensorPtr<T>&, const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&, \
+                                     int64_t);                                                            \
+    template void abs_values<T>(const TensorPtr<T>&, TensorPtr<T>&);
 
 HX_INST_MEM(int32_t)
 HX_INST_MEM(int64_t)

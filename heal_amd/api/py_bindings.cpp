@@ -4,8 +4,10 @@
 // (example_impl/py_bindings.cpp:87-133: <op>_32 / <op>_64 / <op>_float64 and
 // the classes DeviceTensor32 / DeviceTensor64 / DeviceTensorfloat64), so its
 // Python dispatcher can load this module unchanged.  Added: mod_tt / mod_tc /
-// mod_ct (declared by the reference API but not bound there), and backend
-// controls (set_stream, synchronize, plan_cache_size).
+// mod_ct (declared by the reference API but not bound there), the transcript
+// operations the reference runtime names without defining (modneg, abs,
+// modmul_axis_sum, set_const_val, moveaxis, get_slice, pad_single_axis), and
+// backend controls (set_stream, synchronize, plan_cache_size).
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 #include <torch/extension.h>
@@ -36,6 +38,27 @@ static void bind_memory(py::module_& m, const std::string& s) {
     m.def(("squeeze_" + s).c_str(), &squeeze<T>, py::arg("tensor"), py::arg("axis"));
     m.def(("unsqueeze_" + s).c_str(), &unsqueeze<T>, py::arg("tensor"), py::arg("axis"));
     m.def(("make_contiguous_" + s).c_str(), &make_contiguous<T>, py::arg("tensor"));
+    // transcript operations without a reference definition (see lattica_hw_api.h)
+    m.def(("set_const_val_" + s).c_str(), &set_const_val<T>, py::arg("tensor"), py::arg("value"));
+    m.def(("moveaxis_" + s).c_str(), &moveaxis<T>, py::arg("tensor"), py::arg("source"), py::arg("destination"));
+    m.def(("pad_single_axis_" + s).c_str(), &pad_single_axis<T>, py::arg("tensor"), py::arg("axis"),
+          py::arg("before"), py::arg("after"));
+    // spec: one (start, stop, step, is_index) per leading dim, already normalised
+    m.def(
+        ("get_slice_" + s).c_str(),
+        [](const std::shared_ptr<DeviceTensor<T>>& t, const std::vector<std::tuple<int64_t, int64_t, int64_t, bool>>& spec) {
+            std::vector<SliceSpec> v;
+            for (const auto& e : spec) {
+                SliceSpec sp;
+                sp.start = std::get<0>(e);
+                sp.stop = std::get<1>(e);
+                sp.step = std::get<2>(e);
+                sp.index = std::get<3>(e);
+                v.push_back(sp);
+            }
+            return get_slice<T>(t, v);
+        },
+        py::arg("tensor"), py::arg("spec"));
 }
 
 template <typename T>
@@ -57,6 +80,12 @@ static void bind_compute(py::module_& m, const std::string& s) {
     m.def(("ntt_" + s).c_str(), &ntt<T>);
     m.def(("intt_" + s).c_str(), &intt<T>);
     m.def(("permute_" + s).c_str(), &permute<T>);
+    // transcript operations without a reference definition (see lattica_hw_api.h)
+    m.def(("modneg_tt_" + s).c_str(), &modneg_tt<T>);
+    m.def(("modneg_tc_" + s).c_str(), &modneg_tc<T>);
+    m.def(("modmul_axis_sum_" + s).c_str(), &modmul_axis_sum<T>, py::arg("a"), py::arg("b"), py::arg("p"),
+          py::arg("result"), py::arg("axis"));
+    m.def(("abs_" + s).c_str(), &abs_values<T>);
 }
 
 PYBIND11_MODULE(lattica_hw, m) {

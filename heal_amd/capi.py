@@ -69,6 +69,11 @@ _PROTOS = {
     "hx_mod_tc": [VP, I64, VP, STREAM],
     "hx_mod_ct": [I64, VP, VP, STREAM],
     "hx_axis_modsum": [VP, VP, VP, I64, STREAM],
+    "hx_modneg_tt": [VP, VP, VP, STREAM],
+    "hx_modneg_tc": [VP, I64, VP, STREAM],
+    "hx_modmul_axis_sum": [VP, VP, VP, VP, I64, STREAM],
+    "hx_abs": [VP, VP, STREAM],
+    "hx_set_const": [VP, I64, ctypes.c_double, STREAM],
     "hx_g_decomposition": [VP, VP, U64, U64, STREAM],
     "hx_ntt": [VP, VP, VP, VP, VP, VP, VP, STREAM],
     "hx_intt": [VP, VP, VP, VP, VP, VP, VP, VP, STREAM],
@@ -225,6 +230,39 @@ def axis_modsum(a, p, out, axis: int):
     _need_cuda(a, p, out)
     check(lib.hx_axis_modsum(view(a), view(p), view(out), int(axis), stream_of(out)))
     return out
+
+
+def modneg_tt(a, p, out):
+    _need_cuda(a, p, out)
+    check(lib.hx_modneg_tt(view(a), view(p), view(out), stream_of(out)))
+    return out
+
+
+def modneg_tc(a, p: int, out):
+    _need_cuda(a, out)
+    check(lib.hx_modneg_tc(view(a), int(p), view(out), stream_of(out)))
+    return out
+
+
+def modmul_axis_sum(a, b, p, out, axis: int):
+    _need_cuda(a, b, p, out)
+    check(lib.hx_modmul_axis_sum(view(a), view(b), view(p), view(out), int(axis), stream_of(out)))
+    return out
+
+
+def abs_(a, out):
+    _need_cuda(a, out)
+    check(lib.hx_abs(view(a), view(out), stream_of(out)))
+    return out
+
+
+def set_const(a, value):
+    _need_cuda(a)
+    if a.dtype == torch.float64:
+        check(lib.hx_set_const(view(a), 0, float(value), stream_of(a)))
+    else:
+        check(lib.hx_set_const(view(a), int(value), 0.0, stream_of(a)))
+    return a
 
 
 def g_decomposition(a, out, power: int, base_bits: int):

@@ -8,7 +8,9 @@
 // tensors) is turned into a table of Möller–Granlund reciprocals in LDS once
 // per workgroup, so the per-element cost is one double-width product and one
 // reduction.
+#include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "hx_internal.hpp"
 
@@ -68,7 +70,8 @@ __device__ __forceinline__ ModInfo modinfo_of(int64_t m) {
     return make_modinfo(M);
 }
 
-// OP: 0 = modmul, 1 = modsum, 2 = remainder (a % b: the modulus is b, p unused)
+// OP: 0 = modmul, 1 = modsum, 2 = remainder (a % b: the modulus is b, p unused),
+//     3 = modular negation (the residue of -a in [0, |p|); b unused)
 template <typename T, int OP, int BSRC, int PSRC>
 __global__ __launch_bounds__(kBlock) void k_modop(ElemArgs A) {
     __shared__ ModInfo table[kTableMax];
@@ -95,6 +98,9 @@ __global__ __launch_bounds__(kBlock) void k_modop(ElemArgs A) {
         if (mi.M != 0) {
             if (OP == 2) {
                 r = cpp_rem(mi, av);
+            } else if (OP == 3) {
+                const uint64_t ra = reduce64(mi, uabs64(av));
+                r = (int64_t)(av < 0 ? ra : (ra ? mi.M - ra : 0));
             } else {
                 int64_t bv = (BSRC == SRC_SCALAR) ? to_t<T>(A.b_scalar) : ld<T>(A.b, off[2]);
                 r = (OP == 0) ? cpp_mulmod(mi, av, bv) : cpp_addmod(mi, av, bv);
@@ -210,6 +216,9 @@ void modop(int op, const hx_view* a, const hx_view* b, int64_t bs, const hx_view
     if (op == 0) {
         if (is32) launch_modop_t<int32_t, 0>(A, bsrc, psrc, st);
         else launch_modop_t<int64_t, 0>(A, bsrc, psrc, st);
+    } else if (op == 3) {
+        if (is32) launch_modop_t<int32_t, 3>(A, SRC_SCALAR, psrc, st);
+        else launch_modop_t<int64_t, 3>(A, SRC_SCALAR, psrc, st);
     } else {
         if (is32) launch_modop_t<int32_t, 1>(A, bsrc, psrc, st);
         else launch_modop_t<int64_t, 1>(A, bsrc, psrc, st);
@@ -331,6 +340,169 @@ void axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, int6
     if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_axis_modsum<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
     else hipLaunchKernelGGL(k_axis_modsum<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
     check_hip(hipGetLastError(), "axis_modsum launch");
+}
+
+// --------------------------------------------- modmul_axis_sum (fused)
+// out = axis_modsum(modmul_ttt(a, b, p), p, axis): for every output element
+// the products (T_DP(a) * T_DP(b)) % p along `axis` are summed with the
+// reference's step semantics (sum = (sum + prod) % p, T arithmetic) without
+// materialising the product tensor.
+struct MasArgs {
+    IterSpace it;  // over result: 0 = result, 1 = a, 2 = b (axis removed)
+    void* out;
+    const void* a;
+    const void* b;
+    const void* p;
+    int64_t axis_len;
+    int64_t a_axis_stride, b_axis_stride;
+    int64_t p_stride;
+    int32_t table;
+    int32_t k;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_modmul_axis_sum(MasArgs A) {
+    __shared__ ModInfo table[kTableMax];
+    if (A.table) {
+        for (int i = threadIdx.x; i < A.k; i += blockDim.x) table[i] = modinfo_of(ld<T>(A.p, (int64_t)i * A.p_stride));
+        __syncthreads();
+    }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        int64_t off[3], last = 0;
+        iter_offsets<3>(A.it, idx, off, &last);
+        const ModInfo mi = A.table ? table[last] : modinfo_of(ld<T>(A.p, last * A.p_stride));
+        int64_t sum = 0;
+        if (mi.M != 0) {
+            for (int64_t r = 0; r < A.axis_len; ++r) {
+                const int64_t prod =
+                    to_t<T>(cpp_mulmod(mi, ld<T>(A.a, off[1] + r * A.a_axis_stride), ld<T>(A.b, off[2] + r * A.b_axis_stride)));
+                const int64_t s2 = (int64_t)((uint64_t)sum + (uint64_t)prod);
+                int64_t red;
+                if (s2 >= 0 && (uint64_t)s2 < mi.M) red = s2;
+                else if (s2 >= 0 && mi.M <= (1ull << 62) && (uint64_t)s2 < 2 * mi.M) red = s2 - (int64_t)mi.M;
+                else red = cpp_rem(mi, s2);
+                sum = to_t<T>(red);
+            }
+        }
+        st_<T>(A.out, off[0], sum);
+    }
+}
+
+void modmul_axis_sum(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, int64_t axis,
+                     hipStream_t st) {
+    check_view(a, "a");
+    check_view(b, "b");
+    check_view(p, "p");
+    check_view(result, "result");
+    require_int(result, result, "result");
+    require_int(a, result, "a");
+    require_int(b, result, "b");
+    require_int(p, result, "p");
+    if (p->ndim != 1) raise(HX_ERR_INVALID_ARGUMENT, "p must be a 1D tensor of shape [k]");
+    // full (broadcast) shape of a and b, right-aligned
+    const int nd = std::max(a->ndim, b->ndim);
+    int64_t full[HX_MAX_DIMS], sa[HX_MAX_DIMS] = {}, sb[HX_MAX_DIMS] = {};
+    for (int i = 0; i < nd; ++i) {
+        const int ia = i - (nd - a->ndim), ib = i - (nd - b->ndim);
+        const int64_t da = ia >= 0 ? a->dims[ia] : 1, db = ib >= 0 ? b->dims[ib] : 1;
+        if (da != db && da != 1 && db != 1) raise(HX_ERR_INVALID_ARGUMENT, "a and b are not broadcast-compatible");
+        full[i] = std::max(da, db);
+        sa[i] = (ia >= 0 && da != 1) ? a->strides[ia] : 0;
+        sb[i] = (ib >= 0 && db != 1) ? b->strides[ib] : 0;
+    }
+    if (axis < 0 || axis >= nd - 1)
+        raise(HX_ERR_INVALID_ARGUMENT, "axis must be in range [0, ndim - 2] (can't reduce across last axis)");
+    if (full[nd - 1] != p->dims[0]) raise(HX_ERR_INVALID_ARGUMENT, "Last dimension of a must match shape of p");
+    if (result->ndim != nd - 1) raise(HX_ERR_INVALID_ARGUMENT, "result must have the shape of a without axis");
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0, j = 0; i < nd; ++i) {
+        if (i == axis) continue;
+        if (result->dims[j] != full[i]) raise(HX_ERR_INVALID_ARGUMENT, "result must have the shape of a without axis");
+        strides[0][j] = result->strides[j];
+        strides[1][j] = sa[i];
+        strides[2][j] = sb[i];
+        ++j;
+    }
+    MasArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(result->ndim, result->dims, 3, strides, true);
+    if (A.it.total == 0) return;
+    A.out = result->data;
+    A.a = a->data;
+    A.b = b->data;
+    A.p = p->data;
+    A.axis_len = full[axis];
+    A.a_axis_stride = sa[axis];
+    A.b_axis_stride = sb[axis];
+    A.p_stride = p->strides[0];
+    A.k = (int32_t)p->dims[0];
+    A.table = p->dims[0] <= kTableMax;
+    unsigned g = grid_for(A.it.total, kBlock);
+    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_modmul_axis_sum<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_modmul_axis_sum<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    check_hip(hipGetLastError(), "modmul_axis_sum launch");
+}
+
+// ------------------------------------------------------------ abs, fill
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_abs(ElemArgs A) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        int64_t off[2], last = 0;
+        iter_offsets<2>(A.it, idx, off, &last);
+        const T v = ((const T*)A.a)[off[1]];
+        ((T*)A.out)[off[0]] = v < 0 ? (T)(0 - (typename std::make_unsigned<T>::type)v) : v;  // wraps at the minimum
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(ElemArgs A, T value) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
+        int64_t off[1], last = 0;
+        iter_offsets<1>(A.it, idx, off, &last);
+        ((T*)A.out)[off[0]] = value;
+    }
+}
+
+void abs_values(const hx_view* a, const hx_view* result, hipStream_t st) {
+    check_view(a, "a");
+    check_view(result, "result");
+    require_int(result, result, "result");
+    require_int(a, result, "a");
+    if (!dims_equal(a, result)) raise(HX_ERR_INVALID_ARGUMENT, "a must have exactly the same shape as result.");
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0; i < result->ndim; ++i) {
+        strides[0][i] = result->strides[i];
+        strides[1][i] = a->strides[i];
+    }
+    ElemArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(result->ndim, result->dims, 2, strides, false);
+    if (A.it.total == 0) return;
+    A.out = result->data;
+    A.a = a->data;
+    unsigned g = grid_for(A.it.total, kBlock);
+    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_abs<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_abs<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    check_hip(hipGetLastError(), "abs launch");
+}
+
+void set_const(const hx_view* a, int64_t ivalue, double fvalue, hipStream_t st) {
+    check_view(a, "a");
+    int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
+    for (int i = 0; i < a->ndim; ++i) strides[0][i] = a->strides[i];
+    ElemArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.it = make_iter(a->ndim, a->dims, 1, strides, false);
+    if (A.it.total == 0) return;
+    A.out = a->data;
+    unsigned g = grid_for(A.it.total, kBlock);
+    if (a->dtype == HX_INT32) hipLaunchKernelGGL(k_fill<int32_t>, dim3(g), dim3(kBlock), 0, st, A, (int32_t)ivalue);
+    else if (a->dtype == HX_INT64) hipLaunchKernelGGL(k_fill<int64_t>, dim3(g), dim3(kBlock), 0, st, A, ivalue);
+    else hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kBlock), 0, st, A, fvalue);
+    check_hip(hipGetLastError(), "fill launch");
 }
 
 // -------------------------------------------------------- g_decomposition
@@ -590,6 +762,41 @@ int hx_mod_tc(const hx_view* a, int64_t b, const hx_view* result, hx_stream stre
 int hx_mod_ct(int64_t a, const hx_view* b, const hx_view* result, hx_stream stream) {
     HX_API_BEGIN
     hx::modrem(nullptr, false, a, b, true, 0, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_modneg_tt(const hx_view* a, const hx_view* p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!p) raise(HX_ERR_INVALID_ARGUMENT, "p pointer must not be null.");
+    hx::modop(3, a, nullptr, 0, p, 0, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_modneg_tc(const hx_view* a, int64_t p, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    hx::modop(3, a, nullptr, 0, nullptr, p, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_modmul_axis_sum(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, int64_t axis,
+                       hx_stream stream) {
+    HX_API_BEGIN
+    if (!a || !b || !p || !result) raise(HX_ERR_INVALID_ARGUMENT, "tensor pointers must not be null.");
+    hx::modmul_axis_sum(a, b, p, result, axis, as_stream(stream));
+    HX_API_END
+}
+
+int hx_abs(const hx_view* a, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    if (!a || !result) raise(HX_ERR_INVALID_ARGUMENT, "tensor pointers must not be null.");
+    hx::abs_values(a, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_set_const(const hx_view* a, int64_t ivalue, double fvalue, hx_stream stream) {
+    HX_API_BEGIN
+    if (!a) raise(HX_ERR_INVALID_ARGUMENT, "tensor pointer must not be null.");
+    hx::set_const(a, ivalue, fvalue, as_stream(stream));
     HX_API_END
 }
 

@@ -9,7 +9,10 @@ unsqueeze, contiguous, and ntt(a, perm, perm_pairs, q_list, log2p, mu_list,
 psi_arr, out, tile, skip_perm).  The reference declares but does not implement
 intt, take_along_axis, apply_g_decomp and _mod_* (device_interface.py:44-76);
 here they are implemented with the argument order of the corresponding C++ API
-call, since no transcript in the reference fixes another.
+call, since no transcript in the reference fixes another.  The same holds for
+the ops it names without any definition (_modneg_*, modmul_axis_sum, abs,
+set_const_val, moveaxis, pad_single_axis, get_slice): their argument orders
+and semantics are this library's (lattica_hw_api.h).
 """
 from __future__ import annotations
 
@@ -125,18 +128,59 @@ class HealDispatcher:
         return out
 
     def modmul_axis_sum(self, a, b, q_list, out, axis):
-        """(a * b) % q summed over `axis` (a modmul into a temporary, then axis_modsum)."""
-        prod = self._f("allocate_on_hardware", a)(self._broadcast_shape(a.shape, b.shape))
-        self._f("modmul_ttt", a)(a, b, q_list, prod)
-        self._f("axis_modsum", a)(prod, q_list, out, axis)
+        """axis_modsum(modmul_ttt(a, b, q), q, axis) in one kernel (no product tensor)."""
+        self._f("modmul_axis_sum", a)(a, b, q_list, out, axis)
         return out
 
-    @staticmethod
-    def _broadcast_shape(x, y):
-        n = max(len(x), len(y))
-        x = [1] * (n - len(x)) + list(x)
-        y = [1] * (n - len(y)) + list(y)
-        return [max(i, j) for i, j in zip(x, y)]
+    # -- transcript ops the reference names without defining (device_interface.py:
+    #    47-48, 75-107); semantics as in lattica_hw_api.h
+    def _modneg_tt(self, a, p, out):
+        self._f("modneg_tt", out)(a, p, out)
+        return out
+
+    def _modneg_tc(self, a, p, out):
+        self._f("modneg_tc", out)(a, p, out)
+        return out
+
+    def abs(self, a, out):
+        self._f("abs", out)(a, out)
+        return out
+
+    def set_const_val(self, a, value):
+        self._f("set_const_val", a)(a, value)
+        return a
+
+    def moveaxis(self, a, source, destination):
+        return self._f("moveaxis", a)(a, source, destination)
+
+    def pad_single_axis(self, a, axis, before, after):
+        return self._f("pad_single_axis", a)(a, axis, before, after)
+
+    def get_slice(self, a, key):
+        """Basic indexing a[key] (ints, slices with positive step, one Ellipsis): a view."""
+        shape = list(a.shape)
+        key = list(key) if isinstance(key, (tuple, list)) else [key]
+        if sum(k is Ellipsis for k in key) > 1:
+            raise IndexError("an index can only have a single ellipsis")
+        if Ellipsis in key:
+            i = key.index(Ellipsis)
+            key = key[:i] + [slice(None)] * (len(shape) - (len(key) - 1)) + key[i + 1:]
+        if len(key) > len(shape):
+            raise IndexError("too many indices for tensor")
+        spec = []
+        for k, dim in zip(key, shape):
+            if isinstance(k, slice):
+                start, stop, step = k.indices(dim)
+                if step <= 0:
+                    raise ValueError("slice step must be positive")
+                stop = max(stop, start)
+                spec.append((start, stop, step, False))
+            else:
+                i = int(k)
+                if i < -dim or i >= dim:
+                    raise IndexError(f"index {i} is out of bounds for dimension with size {dim}")
+                spec.append((i + dim if i < 0 else i, 0, 1, True))
+        return self._f("get_slice", a)(a, spec)
 
     # -- special ops ---------------------------------------------------------------
     def ntt(self, a, perm, perm_pairs, q_list, log2p, mu_list, psi_arr, out, tile, skip_perm):
@@ -162,7 +206,4 @@ class HealDispatcher:
         return out
 
     def __getattr__(self, name):
-        # ops the transcript format names whose semantics no part of the
-        # reference defines (moveaxis, pad_single_axis, get_slice, abs,
-        # set_const_val, _modneg_*)
         raise NotImplementedError(f"transcript op {name!r} is not part of the HEAL hardware API")

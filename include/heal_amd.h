@@ -111,6 +111,25 @@ int hx_mod_ct(int64_t a, const hx_view* b, const hx_view* result, hx_stream stre
  * (reference include/axis_modsum.h:24-30, example_impl/axis_modsum_impl.cpp:12-76). */
 int hx_axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, int64_t axis, hx_stream stream);
 
+/* Transcript operations that the reference's runtime names but no reference
+ * file defines (python_execution/lattica_heal_runtime/device_interface.py:47-48,
+ * 78-85: _modneg_tt/_modneg_tc, modmul_axis_sum, abs, set_const_val).
+ * Their semantics here are this library's own:
+ *   modneg:          result = the residue of -a modulo |p| in [0, |p|) (0 for p = 0);
+ *                    p broadcasts as in modmul;
+ *   modmul_axis_sum: result = axis_modsum(modmul_ttt(a, b, p), p, axis) with the
+ *                    reference's step semantics of both ops, without the product
+ *                    tensor; a and b broadcast against each other (right-aligned),
+ *                    axis counts in the broadcast shape;
+ *   abs:             result = |a| (two's complement wrap at the minimum);
+ *   set_const:       every element of a := ivalue (integer dtypes) or fvalue (float64). */
+int hx_modneg_tt(const hx_view* a, const hx_view* p, const hx_view* result, hx_stream stream);
+int hx_modneg_tc(const hx_view* a, int64_t p, const hx_view* result, hx_stream stream);
+int hx_modmul_axis_sum(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, int64_t axis,
+                       hx_stream stream);
+int hx_abs(const hx_view* a, const hx_view* result, hx_stream stream);
+int hx_set_const(const hx_view* a, int64_t ivalue, double fvalue, hx_stream stream);
+
 /* result[..., d] = d-th base-2^base_bits digit of a (unsigned digits)
  * (reference include/g_decomposition.h:24-30, example_impl/g_decomposition_impl.cpp:10-67).
  * Elements whose value is not exhausted after `power` digits increment the

@@ -101,4 +101,4 @@ def test_reference_dispatch_conventions(engine):
     B.d2h(r, torch.tensor([1, 9, 3]))
     run_transcript(engine, B.transcript().transcript, verify=True)
     with pytest.raises(NotImplementedError):
-        engine.moveaxis(None, 0, 1)
+        engine.no_such_op(None, 0, 1)
