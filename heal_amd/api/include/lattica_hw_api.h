@@ -128,6 +128,18 @@ template <typename T>
 std::shared_ptr<DeviceTensor<T>> pad_single_axis(const std::shared_ptr<DeviceTensor<T>>& a, int64_t axis,
                                                  int64_t before, int64_t after);
 
+// ---- views and transfers for programs replayed many times (no counterpart in
+// the reference)
+// A new tensor object over a's buffer with the given metadata (no copy); the
+// addressed span must lie inside the buffer.
+template <typename T>
+std::shared_ptr<DeviceTensor<T>> alias_view(const std::shared_ptr<DeviceTensor<T>>& a, const std::vector<int64_t>& dims,
+                                            const std::vector<int64_t>& strides, int64_t offset);
+// Writes a host tensor of dtype T with exactly a's dims and strides into a's
+// buffer (the span it addresses) and marks the buffer as written.
+template <typename T>
+void copy_from_host(const std::shared_ptr<DeviceTensor<T>>& a, const torch::Tensor& tensor);
+
 // ---- backend controls (no counterpart in the reference)
 // Stream that subsequent ops of the calling thread are enqueued on (a
 // hipStream_t; nullptr = the legacy default stream).

@@ -698,6 +698,49 @@ TensorPtr<T> pad_single_axis(const TensorPtr<T>& a, int64_t axis, int64_t before
     return out;
 }
 
+template <typename T>
+TensorPtr<T> alias_view(const TensorPtr<T>& a, const std::vector<int64_t>& dims, const std::vector<int64_t>& strides,
+                        int64_t offset) {
+    require(a, "a");
+    if (dims.size() != strides.size()) throw std::invalid_argument("dims and strides differ in length");
+    int64_t lo = offset, hi = offset;
+    for (size_t i = 0; i < dims.size(); ++i) {
+        if (dims[i] < 0) throw std::invalid_argument("dimensions must be non-negative");
+        if (dims[i] == 0) {
+            lo = hi = offset;
+            break;
+        }
+        if (strides[i] >= 0) hi += (dims[i] - 1) * strides[i];
+        else lo += (dims[i] - 1) * strides[i];
+    }
+    const int64_t cap = a->buffer ? (int64_t)(a->buffer->bytes / sizeof(T)) : 0;
+    if (lo < 0 || (hi >= cap && cap > 0) || (cap == 0 && hi > 0)) throw std::out_of_range("view outside its buffer");
+    auto v = std::make_shared<DeviceTensor<T>>();
+    v->dims = dims;
+    v->strides = strides;
+    v->buffer = a->buffer;
+    v->offset = offset;
+    return v;
+}
+
+template <typename T>
+void copy_from_host(const TensorPtr<T>& a, const torch::Tensor& tensor) {
+    require(a, "a");
+    if (tensor.scalar_type() != torch::CppTypeToScalarType<T>())
+        throw std::runtime_error("Tensor dtype does not match template parameter T.");
+    if (std::vector<int64_t>(tensor.sizes().begin(), tensor.sizes().end()) != a->dims ||
+        std::vector<int64_t>(tensor.strides().begin(), tensor.strides().end()) != a->strides)
+        throw std::invalid_argument("host tensor must have the device tensor's dims and strides");
+    const int64_t span = span_elems(a->dims, a->strides);
+    if (span) {
+        if (tensor.is_cuda())
+            check(hx_memcpy_d2d(a->data_ptr(), tensor.data_ptr(), (size_t)span * sizeof(T), t_stream));
+        else
+            check(hx_memcpy_h2d(a->data_ptr(), tensor.data_ptr(), (size_t)span * sizeof(T), t_stream));
+    }
+    bump(a);
+}
+
 // ------------------------------------------------------------------ instantiations
 #define HX_INST_MEM(T)                                                                                          \
     template std::shared_ptr<DeviceTensor<T>> allocate_on_hardware<T>(const std::vector<int64_t>&);            \
@@ -710,7 +753,10 @@ TensorPtr<T> pad_single_axis(const TensorPtr<T>& a, int64_t axis, int64_t before
     template void set_const_val<T>(const TensorPtr<T>&, T);                                                   \
     template TensorPtr<T> moveaxis<T>(const TensorPtr<T>&, int64_t, int64_t);                                 \
     template TensorPtr<T> get_slice<T>(const TensorPtr<T>&, const std::vector<SliceSpec>&);                  \
-    template TensorPtr<T> pad_single_axis<T>(const TensorPtr<T>&, int64_t, int64_t, int64_t);
+    template TensorPtr<T> pad_single_axis<T>(const TensorPtr<T>&, int64_t, int64_t, int64_t);                \
+    template TensorPtr<T> alias_view<T>(const TensorPtr<T>&, const std::vector<int64_t>&,                     \
+                                        const std::vector<int64_t>&, int64_t);                                \
+    template void copy_from_host<T>(const TensorPtr<T>&, const torch::Tensor&);
 
 #define HX_INST_OPS(T)                                                                                     \
     template void modmul_ttt<T>(const TensorPtr<T>&, const TensorPtr<T>&, const TensorPtr<T>&, TensorPtr<T>&); \

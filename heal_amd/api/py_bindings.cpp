@@ -26,6 +26,9 @@ static void bind_tensor_class(py::module_& m, const std::string& suffix) {
         .def("reshape", &DT::reshape)
         .def_property_readonly("shape", [](const DT& t) { return t.dims; })
         .def_property_readonly("strides", [](const DT& t) { return t.strides; })
+        .def_property_readonly("offset", [](const DT& t) { return t.offset; })
+        .def_property_readonly("buffer_id", [](const DT& t) { return t.buffer ? t.buffer->id : uint64_t(0); })
+        .def_property_readonly("buffer_version", [](const DT& t) { return t.buffer ? t.buffer->version : uint64_t(0); })
         .def("is_contiguous", &DT::is_contiguous);
 }
 
@@ -43,6 +46,9 @@ static void bind_memory(py::module_& m, const std::string& s) {
     m.def(("moveaxis_" + s).c_str(), &moveaxis<T>, py::arg("tensor"), py::arg("source"), py::arg("destination"));
     m.def(("pad_single_axis_" + s).c_str(), &pad_single_axis<T>, py::arg("tensor"), py::arg("axis"),
           py::arg("before"), py::arg("after"));
+    m.def(("alias_view_" + s).c_str(), &alias_view<T>, py::arg("tensor"), py::arg("dims"), py::arg("strides"),
+          py::arg("offset"));
+    m.def(("copy_from_host_" + s).c_str(), &copy_from_host<T>, py::arg("device_tensor"), py::arg("tensor"));
     // spec: one (start, stop, step, is_index) per leading dim, already normalised
     m.def(
         ("get_slice_" + s).c_str(),

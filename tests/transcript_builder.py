@@ -36,6 +36,13 @@ class Builder:
                                                DeviceOpArg(A.DEVICE_TENSOR, p))))
         return p
 
+    def zeros(self, shape, dtype=torch.int64):
+        p = self._ptr(dtype)
+        self.ops.append((K.DEVICE_OP, DeviceOp("zeros", [DeviceOpArg(A.SHAPE, [DeviceOpArg(A.INT, s) for s in shape]),
+                                                         DeviceOpArg(A.TENSOR_TYPE, dtype)],
+                                               DeviceOpArg(A.DEVICE_TENSOR, p))))
+        return p
+
     def op(self, name, *args, out=None):
         conv = []
         for a in args:
