@@ -88,6 +88,9 @@ _PROTOS = {
     "hx_keyswitch_prepare_key": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
     "hx_keyswitch_exec": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, VP, STREAM],
     "hx_keyswitch_plan_destroy": [ctypes.c_void_p],
+    "hx_keyswitch_exec_ex": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, I64, I64, ctypes.c_void_p, VP,
+                             STREAM],
+    "hx_keyswitch_row_map": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
 }
 
 
@@ -359,6 +362,23 @@ class KeySwitchPlan:
         check(lib.hx_keyswitch_exec(self._h, view(x), key.data_ptr(), key.numel() * key.element_size(), view(out),
                                     stream_of(out)))
         return out
+
+    def exec_ex(self, x, key, out, key_limbs: int, key_count: int, row_map=None):
+        """Key-switch with a shared or per-instance key of key_limbs limbs and an
+        optional row map (int32 [m], from row_map()) for the Galois permutation."""
+        _need_cuda(x, key, out)
+        check(lib.hx_keyswitch_exec_ex(self._h, view(x), key.data_ptr(), key.numel() * key.element_size(),
+                                       int(key_limbs), int(key_count),
+                                       row_map.data_ptr() if row_map is not None else None, view(out),
+                                       stream_of(out)))
+        return out
+
+    def row_map(self, perm):
+        """int32 [m] row map of one Galois permutation perm [m]."""
+        _need_cuda(perm)
+        rm = torch.empty(perm.shape[-1], dtype=torch.int32, device=perm.device)
+        check(lib.hx_keyswitch_row_map(self._h, view(perm), rm.data_ptr(), stream_of(perm)))
+        return rm
 
     def close(self):
         if self._h:

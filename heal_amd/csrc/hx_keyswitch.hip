@@ -41,7 +41,9 @@ namespace {
 struct InnerArgs {
     const uint64_t* src;  // K1 output: rows of D*k*B, column d + D*(t + k*b)
     uint64_t* dst;        // K3 input:  rows of 2*k*B, column c + 2*(t + k*b)
-    const int64_t* key;   // [D][B][2][k][m] in network order
+    const int64_t* key;   // [D][key_b][2][key_k][m], window-local order (k_ks_prepare_key)
+    int64_t key_k;        // limbs the key was prepared with (>= k; the first k are used)
+    int64_t key_b;        // keys in the buffer: 1 = one key for every instance, else B
     const LimbInfo* limbs;
     const uint64_t* tw_f;
     const uint64_t* tw_i;
@@ -144,8 +146,9 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
     if (valid) {
         const ModInfo mi = limb_modinfo(L);
         const bool one_reduction = (uint64_t)D <= ~0ull / q;
-        const int64_t* kp = A.key + ((b * 2 + c) * A.k + t) * A.m + row0 + lane;
-        const int64_t dstride = A.B * 2 * A.k * A.m;
+        const int64_t kb = A.key_b == 1 ? 0 : b;
+        const int64_t* kp = A.key + ((kb * 2 + c) * A.key_k + t) * A.m + row0 + lane;
+        const int64_t dstride = A.key_b * 2 * A.key_k * A.m;
         const uint64_t* xf = lds + (int64_t)(D * gi) * SP;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
@@ -345,6 +348,49 @@ void keyswitch_plan_destroy(hx_keyswitch_plan* plan) {
     delete plan;
 }
 
+// Inverse of one Galois permutation perms [m] (or [1, m]) for keyswitch_exec's
+// row_map: row_map[perms[n]] = n; perms must be a permutation of [0, m).
+void keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int32_t* row_map, hipStream_t st) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    if (!row_map) raise(HX_ERR_INVALID_ARGUMENT, "row_map pointer must not be null.");
+    check_view(perms, "perms");
+    if (perms->dtype != HX_INT32 && perms->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, "perms must be integer");
+    const int64_t m = plan->m;
+    const bool ok1 = perms->ndim == 1 && perms->dims[0] == m;
+    const bool ok2 = perms->ndim == 2 && perms->dims[0] == 1 && perms->dims[1] == m;
+    if (!ok1 && !ok2) raise(HX_ERR_INVALID_ARGUMENT, "perms must have shape [m] or [1, m]");
+    hx_view v = *perms;
+    if (ok1) {  // as [1, m]
+        v.ndim = 2;
+        v.dims[1] = m;
+        v.strides[1] = perms->strides[0];
+        v.dims[0] = 1;
+        v.strides[0] = 0;
+    }
+    const size_t cnt_bytes = ((size_t)m * 4 + 255) / 256 * 256;
+    void* mem = nullptr;
+    check_hip(hipMallocAsync(&mem, cnt_bytes + 256, st), "hipMallocAsync(row map)");
+    int32_t* counts = (int32_t*)mem;
+    unsigned long long* flags = (unsigned long long*)((char*)mem + cnt_bytes);
+    unsigned long long h[2] = {0, 0};
+    try {
+        check_hip(hipMemsetAsync(mem, 0, cnt_bytes + 256, st), "hipMemsetAsync");
+        PlanSrc ps = plan_src(&v);
+        unsigned g = grid_for(m, 256, 8192);
+        hipLaunchKernelGGL(k_ks_gal_count, dim3(g), dim3(256), 0, st, ps, (int64_t)1, m, counts, flags);
+        hipLaunchKernelGGL(k_ks_gal_map, dim3(g), dim3(256), 0, st, ps, (int64_t)1, m, counts, row_map, flags);
+        check_hip(hipGetLastError(), "row map launch");
+        check_hip(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipFreeAsync(mem, st);
+        throw;
+    }
+    check_hip(hipFreeAsync(mem, st), "hipFreeAsync(row map)");
+    if (h[0]) raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
+    if (h[1]) raise(HX_ERR_INVALID_ARGUMENT, "perms must be a permutation");
+}
+
 static void require_i64(const hx_view* v, const char* name, int ndim) {
     check_view(v, name);
     if (v->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must be int64");
@@ -378,14 +424,18 @@ void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, vo
 }
 
 void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
-                    const hx_view* out, hipStream_t st) {
+                    int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(x, "x", 3);
     require_i64(out, "out", 4);
     if (!key) raise(HX_ERR_INVALID_ARGUMENT, "key pointer must not be null.");
     const int64_t B = x->dims[0], m = plan->m, k = plan->k;
     const int D = plan->D;
-    if (B > 0 && key_bytes < keyswitch_key_bytes(plan, B))
+    if (key_limbs < 0) key_limbs = k;
+    if (key_count < 0) key_count = B;
+    if (key_limbs < k) raise(HX_ERR_INVALID_ARGUMENT, "the key has fewer limbs than the plan");
+    if (key_count != 1 && key_count != B) raise(HX_ERR_INVALID_ARGUMENT, "key_count must be 1 or the batch size");
+    if (B > 0 && key_bytes < (size_t)D * (size_t)key_count * 2 * (size_t)key_limbs * (size_t)m * 8)
         raise(HX_ERR_INVALID_ARGUMENT, "key buffer smaller than a key for this batch");
     if (x->dims[1] != m || x->dims[2] != k) raise(HX_ERR_INVALID_ARGUMENT, "x must have shape [B, m, k]");
     if (out->dims[0] != B || out->dims[1] != 2 || out->dims[2] != m || out->dims[3] != k)
@@ -452,6 +502,8 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     A.src = tmp1;
     A.dst = tmp2;
     A.key = (const int64_t*)key;
+    A.key_k = key_limbs;
+    A.key_b = key_count;
     A.limbs = F->limbs;
     A.tw_f = F->tw;
     A.tw_i = I->tw;
@@ -502,7 +554,8 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     P3.stage0 = plan->hlo;
     P3.bot_count = int64_t(1) << plan->hlo;
     P3.scale_out = 1;
-    P3.dst_map_limb = plan->galmap;
+    if (row_map) P3.dst_map = row_map;  // one Galois permutation for every limb
+    else P3.dst_map_limb = plan->galmap;
     P3.limbs = I->limbs;
     P3.tw = I->tw;
     P3.m = m;
@@ -541,7 +594,22 @@ int hx_keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, 
 int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                       const hx_view* out, hx_stream stream) {
     HX_API_BEGIN
-    keyswitch_exec(plan, x, key, key_bytes, out, as_stream(stream));
+    keyswitch_exec(plan, x, key, key_bytes, -1, -1, nullptr, out, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                         int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
+                         hx_stream stream) {
+    HX_API_BEGIN
+    if (key_limbs < 1 || key_count < 1) raise(HX_ERR_INVALID_ARGUMENT, "key_limbs and key_count must be positive");
+    keyswitch_exec(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int32_t* row_map, hx_stream stream) {
+    HX_API_BEGIN
+    keyswitch_row_map(plan, perms, row_map, as_stream(stream));
     HX_API_END
 }
 

@@ -193,6 +193,21 @@ int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const voi
                       const hx_view* out, hx_stream stream);
 int hx_keyswitch_plan_destroy(hx_keyswitch_plan* plan);
 
+/* hx_keyswitch_exec for serving many rotations with one plan per (m, k):
+ *   key_limbs  limbs the key was prepared with (>= the plan's k; the first k
+ *              are used), by a plan of the same m, moduli prefix and NTT perm;
+ *   key_count  1: one key for every instance of the batch; else the batch size;
+ *   row_map    int32 [m] device array from hx_keyswitch_row_map: the Galois
+ *              permutation of every limb (NULL: the plan's own perms).
+ * key_bytes must cover dnum * key_count * 2 * key_limbs * m * 8 bytes.
+ * No counterpart in the reference (a vendor extension of the key-switch above). */
+int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                         int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
+                         hx_stream stream);
+/* row_map[perms[n]] = n for one permutation perms [m] (or [1, m]) of [0, m);
+ * validates perms (synchronises the stream). */
+int hx_keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int32_t* row_map, hx_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
