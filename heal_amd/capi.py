@@ -91,6 +91,11 @@ _PROTOS = {
     "hx_keyswitch_exec_ex": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, I64, I64, ctypes.c_void_p, VP,
                              STREAM],
     "hx_keyswitch_row_map": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
+    "hx_keyswitch_group_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), VP,
+                                  ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), VP, ctypes.c_int,
+                                  STREAM],
+    "hx_keyswitch_group_exec": [ctypes.c_void_p, STREAM],
+    "hx_keyswitch_group_destroy": [ctypes.c_void_p],
 }
 
 
@@ -383,6 +388,50 @@ class KeySwitchPlan:
     def close(self):
         if self._h:
             check(lib.hx_keyswitch_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _view_struct(t):
+    v = HxView()
+    v.data = t.data_ptr() if t.numel() else 0
+    v.dtype = _DTYPE[t.dtype]
+    v.ndim = t.dim()
+    for i, (d, st) in enumerate(zip(t.shape, t.stride())):
+        v.dims[i] = d
+        v.strides[i] = st
+    return v
+
+
+class KeySwitchGroup:
+    """Key-switches of mixed (m, k) run together (hx_keyswitch_group_*): member i
+    is plans[i] on xs[i] with keys[i] into outs[i], fixed at creation."""
+
+    def __init__(self, plans, xs, keys, outs):
+        n = len(plans)
+        if not (len(xs) == len(keys) == len(outs) == n):
+            raise ValueError("plans, xs, keys and outs differ in length")
+        _need_cuda(*xs, *keys, *outs)
+        self._keep = (list(plans), list(xs), list(keys), list(outs))
+        ph = (ctypes.c_void_p * n)(*[p._h.value for p in plans])
+        xv = (HxView * n)(*[_view_struct(x) for x in xs])
+        ov = (HxView * n)(*[_view_struct(o) for o in outs])
+        kp = (ctypes.c_void_p * n)(*[k.data_ptr() for k in keys])
+        kb = (ctypes.c_size_t * n)(*[k.numel() * k.element_size() for k in keys])
+        self._h = ctypes.c_void_p()
+        check(lib.hx_keyswitch_group_create(ctypes.byref(self._h), ph, xv, kp, kb, ov, n, stream_of(outs[0])))
+
+    def __call__(self):
+        check(lib.hx_keyswitch_group_exec(self._h, stream_of()))
+
+    def close(self):
+        if self._h:
+            check(lib.hx_keyswitch_group_destroy(self._h))
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -18,7 +18,11 @@
 // (hx_keyswitch_prepare_key), which is why the forward and inverse plans must
 // share one permutation.  Every elementwise step uses the reference's C++
 // integer semantics, so the result is bit-identical with the six-call chain.
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
+#include <vector>
 
 #include "hx_ntt_kernels.hpp"
 
@@ -29,6 +33,7 @@ struct hx_keyswitch_plan {
     int64_t m, k;
     int logm, hhi, hlo;
     int D, w;
+    int G;            // limbs per K2 workgroup (tuned)
     void* mem;
     int32_t* galmap;  // [k][m]: output row of coefficient j of limb t
     mutable void* ws;
@@ -70,7 +75,7 @@ __device__ __forceinline__ int64_t seq_rem(const ModInfo& mi, int64_t s) {
 // The prepared key holds window index wB(e, lane) of row block hi at
 // hi*W + e*TL + lane, so a key load of one register covers TL adjacent words.
 template <int H>
-__global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
+__device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_t nblk) {
     constexpr int W = 1 << H;
     constexpr int TL = W / 16;
     constexpr int SP = W + 1;
@@ -79,7 +84,7 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
     const int NF = G * D, NI = 2 * G;
     const int tid = threadIdx.x;
 
-    int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    int64_t tile = xcd_tile(blk, nblk);
     const int64_t hi = tile % A.n_hi;
     tile /= A.n_hi;
     const int64_t tg = tile % A.n_tg;
@@ -202,6 +207,18 @@ __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
     for (int e = 0; e < 16; ++e) dst[(int64_t)(e * TL) * ncols2] = x[e];
 }
 
+template <int H>
+__global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
+    ks_inner<H>(A, blockIdx.x, gridDim.x);
+}
+
+// K2 of several key-switches with the same inner window H in one launch
+template <int H>
+__global__ __launch_bounds__(512) void k_ks_inner_multi(const InnerArgs* list, const int64_t* starts, int count) {
+    const int i = instance_of(starts, count, blockIdx.x);
+    ks_inner<H>(list[i], (int64_t)blockIdx.x - starts[i], starts[i + 1] - starts[i]);
+}
+
 // key[((d*B + b)*2 + c)*k + t][hi*W + e*TL + lane] = evk[d, b, c, rowmap[j], t]
 // with j = hi*W + r16_wB(e, lane), W = 2^H the inner window (H = 6..8)
 struct KeyArgs {
@@ -272,6 +289,8 @@ __global__ void k_ks_compare(const LimbInfo* a, const LimbInfo* b, int64_t k, co
 
 }  // namespace
 
+static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st);
+
 hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_plan* inv, const hx_view* perms,
                                          uint64_t dnum, uint64_t base_bits, hipStream_t st) {
     if (!fwd || !inv) raise(HX_ERR_INVALID_ARGUMENT, "plan pointers must not be null.");
@@ -308,6 +327,7 @@ hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_pl
     plan->hhi = logm - plan->hlo;
     plan->D = (int)dnum;
     plan->w = (int)base_bits;
+    plan->G = 1;
     const size_t map_bytes = ((size_t)k * m * 4 + 255) / 256 * 256;
     check_hip(hipMalloc(&plan->mem, 2 * map_bytes + 256), "hipMalloc(keyswitch plan)");
     plan->galmap = (int32_t*)plan->mem;
@@ -328,9 +348,11 @@ hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_pl
         if (h[0]) raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
         if (h[1]) raise(HX_ERR_INVALID_ARGUMENT, "the fused key-switch needs every row of perms to be a permutation");
         if (h[2]) raise(HX_ERR_INVALID_ARGUMENT, "forward and inverse plans differ in moduli or permutation");
+        keyswitch_tune(plan, st);
     } catch (...) {
         (void)hipStreamSynchronize(st);
         (void)hipFree(plan->mem);
+        if (plan->ws) (void)hipFree(plan->ws);
         delete plan;
         throw;
     }
@@ -423,8 +445,36 @@ void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, vo
     check_hip(hipGetLastError(), "prepare key launch");
 }
 
-void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
-                    int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st) {
+// The three launches of one key-switch, fully described.
+struct KsArgs {
+    PassArgs P1;  // K1
+    InnerArgs A;  // K2
+    PassArgs P3;  // K3
+    int64_t blocks1, blocks2, blocks3;
+    int H1, H2, threads2;
+    size_t lds2;
+};
+
+// K2 launch shape for limb group G
+static void inner_shape(int H, int D, int G, int* threads, size_t* lds) {
+    const int TL = (1 << H) / 16;
+    const int rows = std::max(G * D, 2 * G);
+    *threads = (rows * TL + 63) / 64 * 64;
+    *lds = (size_t)rows * ((1 << H) + 1) * 8;
+}
+
+static bool inner_fits(int H, int D, int G) {
+    int threads;
+    size_t lds;
+    inner_shape(H, D, G, &threads, &lds);
+    return threads <= 512 && lds <= 64 * 1024;
+}
+
+// Validates one key-switch, sizes the plan's workspace (outside graph capture)
+// and fills the launch arguments.  Returns false for an empty batch.
+bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                     int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
+                     hipStream_t st, KsArgs& K) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(x, "x", 3);
     require_i64(out, "out", 4);
@@ -444,8 +494,9 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     int dev = 0;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     if (dev != plan->device) raise(HX_ERR_INVALID_ARGUMENT, "plan belongs to another device");
-    if (B == 0) return;
-    if (!r16_window(plan->hhi)) raise(HX_ERR_RUNTIME, "the fused key-switch needs the radix-16 pass kernels");
+    if (B == 0) return false;
+    if (!r16_window(plan->hhi) || !r16_window(plan->hlo))
+        raise(HX_ERR_RUNTIME, "the fused key-switch needs the radix-16 pass kernels");
     const int64_t ncols1 = (int64_t)D * k * B, ncols2 = 2 * k * B;
     const size_t bytes = (size_t)m * (size_t)(ncols1 + ncols2) * 8;
     if (plan->ws_bytes < bytes) {
@@ -465,7 +516,7 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     const hx_ntt_plan* I = plan->inv;
 
     // K1: digits of x, forward pass over the high bits
-    PassArgs P1;
+    PassArgs& P1 = K.P1;
     std::memset(&P1, 0, sizeof P1);
     P1.src.ptr = x->data;
     P1.src.row_stride = x->strides[1];
@@ -495,10 +546,12 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     P1.tw = F->tw;
     P1.m = m;
     configure_tile(P1, true, 2);
-    launch_pass(P1, st);
+    K.blocks1 = pass_blocks(P1);
+    K.H1 = plan->hhi;
 
     // K2: forward low bits, key inner product, inverse low bits
-    InnerArgs A;
+    InnerArgs& A = K.A;
+    std::memset(&A, 0, sizeof A);
     A.src = tmp1;
     A.dst = tmp2;
     A.key = (const int64_t*)key;
@@ -513,27 +566,18 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     A.logm = plan->logm;
     A.hhi = plan->hhi;
     A.D = D;
-    // limb group: about 12 forward sub-problems per workgroup
-    int G = std::max(1, std::min(8, 12 / D));
-    if (G > k) G = (int)k;
+    const int G = (int)std::min<int64_t>(plan->G, k);
     A.G = G;
     A.n_tg = (k + G - 1) / G;
     A.n_hi = int64_t(1) << plan->hhi;
-    const int H = plan->hlo;
-    const int TL = (1 << H) / 16;
-    const int rows = std::max(G * D, 2 * G);
-    const int threads = (rows * TL + 63) / 64 * 64;
-    const size_t lds = (size_t)rows * ((1 << H) + 1) * 8;
-    if (threads > 512 || lds > 64 * 1024)
+    K.H2 = plan->hlo;
+    inner_shape(K.H2, D, G, &K.threads2, &K.lds2);
+    if (K.threads2 > 512 || K.lds2 > 64 * 1024)
         raise(HX_ERR_INVALID_ARGUMENT, "dnum too large for the fused key-switch tile");
-    const int64_t blocks = A.n_hi * A.n_tg * B;
-    if (H == 8) hipLaunchKernelGGL(k_ks_inner<8>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
-    else if (H == 7) hipLaunchKernelGGL(k_ks_inner<7>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
-    else hipLaunchKernelGGL(k_ks_inner<6>, dim3((unsigned)blocks), dim3(threads), lds, st, A);
-    check_hip(hipGetLastError(), "keyswitch inner launch");
+    K.blocks2 = A.n_hi * A.n_tg * B;
 
     // K3: inverse pass over the high bits, m^-1, Galois scatter into out
-    PassArgs P3;
+    PassArgs& P3 = K.P3;
     std::memset(&P3, 0, sizeof P3);
     P3.src = tmp_addr(tmp2, ncols2);
     P3.dst.ptr = out->data;
@@ -560,7 +604,278 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
     P3.tw = I->tw;
     P3.m = m;
     configure_tile(P3, true, 2);
-    launch_pass(P3, st);
+    K.blocks3 = pass_blocks(P3);
+    return true;
+}
+
+static void launch_inner(const KsArgs& K, hipStream_t st) {
+    const dim3 grid((unsigned)K.blocks2), block((unsigned)K.threads2);
+    if (K.H2 == 8) hipLaunchKernelGGL(k_ks_inner<8>, grid, block, K.lds2, st, K.A);
+    else if (K.H2 == 7) hipLaunchKernelGGL(k_ks_inner<7>, grid, block, K.lds2, st, K.A);
+    else hipLaunchKernelGGL(k_ks_inner<6>, grid, block, K.lds2, st, K.A);
+    check_hip(hipGetLastError(), "keyswitch inner launch");
+}
+
+static void keyswitch_launch(KsArgs& K, hipStream_t st) {
+    launch_pass(K.P1, st);
+    launch_inner(K, st);
+    launch_pass(K.P3, st);
+}
+
+void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                    int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st) {
+    KsArgs K;
+    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, st, K)) keyswitch_launch(K, st);
+}
+
+// ------------------------------------------------------------ autotuning
+// Per ring degree m and dnum: the split of log m into the K1/K3 window (high
+// bits) and the K2 window (low bits); per (m, k, dnum): the K2 limb group G.
+// Every choice computes the same result; they differ in speed only.  The split
+// is chosen once per (m, dnum) in a process, so keys prepared by one plan serve
+// every plan of that ring degree.
+static std::mutex g_tune_mu;
+static std::map<std::pair<int64_t, int>, int> g_tuned_hlo;
+static std::map<std::tuple<int64_t, int64_t, int>, int> g_tuned_group;
+
+static float time_keyswitch(hx_keyswitch_plan* plan, const hx_view* x, void* key, size_t key_bytes, const hx_view* out,
+                            hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    KsArgs K;
+    keyswitch_build(plan, x, key, key_bytes, -1, -1, nullptr, out, st, K);
+    keyswitch_launch(K, st);  // warm-up
+    check_hip(hipEventRecord(e0, st), "hipEventRecord");
+    for (int r = 0; r < 3; ++r) keyswitch_launch(K, st);
+    check_hip(hipEventRecord(e1, st), "hipEventRecord");
+    check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    return ms;
+}
+
+static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
+    const int64_t m = plan->m, k = plan->k;
+    const int D = plan->D, logm = plan->logm;
+    std::vector<int> hlos;
+    for (int h = 6; h <= 8; ++h)
+        if (logm - h >= 6 && logm - h <= 8 && inner_fits(h, D, 1)) hlos.push_back(h);
+    std::vector<int> groups;
+    for (int G = 1; G <= 8; G *= 2)
+        if (G <= std::max<int64_t>(k, 1)) groups.push_back(G);
+    const int default_hlo = logm / 2;
+    int hlo = -1, G = -1;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto a = g_tuned_hlo.find({m, D});
+        if (a != g_tuned_hlo.end()) hlo = a->second;
+        auto b = g_tuned_group.find({m, k, D});
+        if (b != g_tuned_group.end()) G = b->second;
+    }
+    if (hlo >= 0 && G >= 0) {
+        plan->hlo = hlo;
+        plan->hhi = logm - hlo;
+        plan->G = G;
+        return;
+    }
+    // scratch key-switch: B instances of zeros (the timing does not depend on values)
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t(1) << 21) / (m * k)));
+    const size_t xb = (size_t)B * m * k * 8, ob = 2 * xb, kb = (size_t)D * B * 2 * k * m * 8;
+    void* mem = nullptr;
+    check_hip(hipMallocAsync(&mem, xb + ob + kb, st), "hipMallocAsync(tuning)");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        check_hip(hipMemsetAsync(mem, 0, xb + ob + kb, st), "hipMemsetAsync(tuning)");
+        check_hip(hipEventCreate(&e0), "hipEventCreate");
+        check_hip(hipEventCreate(&e1), "hipEventCreate");
+        hx_view xv, ov;
+        std::memset(&xv, 0, sizeof xv);
+        std::memset(&ov, 0, sizeof ov);
+        xv.data = mem;
+        xv.dtype = HX_INT64;
+        xv.ndim = 3;
+        xv.dims[0] = B, xv.dims[1] = m, xv.dims[2] = k;
+        xv.strides[0] = m * k, xv.strides[1] = k, xv.strides[2] = 1;
+        ov.data = (char*)mem + xb;
+        ov.dtype = HX_INT64;
+        ov.ndim = 4;
+        ov.dims[0] = B, ov.dims[1] = 2, ov.dims[2] = m, ov.dims[3] = k;
+        ov.strides[0] = 2 * m * k, ov.strides[1] = m * k, ov.strides[2] = k, ov.strides[3] = 1;
+        void* key = (char*)mem + xb + ob;
+        if (hlo < 0) {
+            float best = 1e30f;
+            for (int h : hlos) {
+                plan->hlo = h;
+                plan->hhi = logm - h;
+                plan->G = std::max(1, std::min(8, 12 / D));
+                while (plan->G > 1 && !inner_fits(h, D, plan->G)) plan->G /= 2;
+                const float t = time_keyswitch(plan, &xv, key, kb, &ov, st, e0, e1);
+                if (t < best || (t == best && h == default_hlo)) best = t, hlo = h;
+            }
+        }
+        plan->hlo = hlo;
+        plan->hhi = logm - hlo;
+        if (G < 0) {
+            float best = 1e30f;
+            for (int g : groups) {
+                if (!inner_fits(hlo, D, g)) continue;
+                plan->G = g;
+                const float t = time_keyswitch(plan, &xv, key, kb, &ov, st, e0, e1);
+                if (t < best) best = t, G = g;
+            }
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFreeAsync(mem, st);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_hip(hipFreeAsync(mem, st), "hipFreeAsync(tuning)");
+    // the workspace grew for the scratch batch; release it
+    if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync(ks workspace)");
+    plan->ws = nullptr;
+    plan->ws_bytes = 0;
+    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    plan->G = G;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned_hlo.emplace(std::make_pair(m, D), hlo);
+    g_tuned_group.emplace(std::make_tuple(m, k, D), G);
+    plan->hlo = g_tuned_hlo[{m, D}];
+    plan->hhi = logm - plan->hlo;
+}
+
+// ------------------------------------------------------------ grouped execution
+// Several key-switches of any (m, k) with fixed buffers: per phase, one launch
+// per window size over device arrays of launch arguments.
+struct GroupLaunch {
+    int phase;  // 1, 2, 3
+    int H;
+    int count;
+    int threads;
+    size_t lds;
+    int64_t blocks;
+    size_t args_off, starts_off;  // byte offsets in the group's device memory
+};
+
+}  // namespace hx
+
+struct hx_keyswitch_group {
+    int device;
+    void* mem;                       // launch arguments and tile offsets
+    std::vector<void*> workspaces;   // one per member (stream-ordered allocations)
+    std::vector<hx::GroupLaunch> launches;
+};
+
+namespace hx {
+
+hx_keyswitch_group* keyswitch_group_create(const hx_keyswitch_plan* const* plans, const hx_view* xs,
+                                           const void* const* keys, const size_t* key_bytes, const hx_view* outs,
+                                           int count, hipStream_t st) {
+    if (count < 1) raise(HX_ERR_INVALID_ARGUMENT, "a group needs at least one key-switch");
+    if (!plans || !xs || !keys || !key_bytes || !outs) raise(HX_ERR_INVALID_ARGUMENT, "array pointers must not be null.");
+    // every member gets a workspace of its own (owned by the group), so members
+    // may share a plan and the plans stay free for other work
+    auto* g = new hx_keyswitch_group;
+    g->mem = nullptr;
+    std::vector<KsArgs> ks;
+    try {
+        check_hip(hipGetDevice(&g->device), "hipGetDevice");
+        for (int i = 0; i < count; ++i) {
+            if (!plans[i]) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+            hx_keyswitch_plan own = *plans[i];
+            own.ws = nullptr;
+            own.ws_bytes = 0;
+            KsArgs K;
+            const bool any = keyswitch_build(&own, &xs[i], keys[i], key_bytes[i], -1, -1, nullptr, &outs[i], st, K);
+            if (own.ws) g->workspaces.push_back(own.ws);
+            if (any) ks.push_back(K);
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        for (void* w : g->workspaces) (void)hipFree(w);
+        delete g;
+        throw;
+    }
+    // host image: per launch [args...][starts...]
+    std::vector<char> image;
+    auto align = [&](size_t a) { image.resize((image.size() + a - 1) / a * a); };
+    for (int phase = 1; phase <= 3; ++phase) {
+        for (int H = 6; H <= 8; ++H) {
+            std::vector<const KsArgs*> sel;
+            for (const KsArgs& K : ks)
+                if ((phase == 2 ? K.H2 : K.H1) == H) sel.push_back(&K);
+            if (sel.empty()) continue;
+            GroupLaunch L;
+            L.phase = phase;
+            L.H = H;
+            L.count = (int)sel.size();
+            L.threads = 0;
+            L.lds = 0;
+            align(256);
+            L.args_off = image.size();
+            const size_t asz = phase == 2 ? sizeof(InnerArgs) : sizeof(PassArgs);
+            image.resize(image.size() + asz * sel.size());
+            std::vector<int64_t> starts(1, 0);
+            for (size_t i = 0; i < sel.size(); ++i) {
+                const KsArgs& K = *sel[i];
+                const void* src = phase == 1 ? (const void*)&K.P1 : (phase == 2 ? (const void*)&K.A : (const void*)&K.P3);
+                std::memcpy(image.data() + L.args_off + i * asz, src, asz);
+                starts.push_back(starts.back() + (phase == 1 ? K.blocks1 : (phase == 2 ? K.blocks2 : K.blocks3)));
+                if (phase == 2) {
+                    L.threads = std::max(L.threads, K.threads2);
+                    L.lds = std::max(L.lds, K.lds2);
+                }
+            }
+            L.blocks = starts.back();
+            align(256);
+            L.starts_off = image.size();
+            image.resize(image.size() + starts.size() * sizeof(int64_t));
+            std::memcpy(image.data() + L.starts_off, starts.data(), starts.size() * sizeof(int64_t));
+            g->launches.push_back(L);
+        }
+    }
+    try {
+        check_hip(hipMalloc(&g->mem, std::max<size_t>(image.size(), 256)), "hipMalloc(group)");
+        check_hip(hipMemcpyAsync(g->mem, image.data(), image.size(), hipMemcpyHostToDevice, st), "hipMemcpyAsync(group)");
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        if (g->mem) (void)hipFree(g->mem);
+        for (void* w : g->workspaces) (void)hipFree(w);
+        delete g;
+        throw;
+    }
+    return g;
+}
+
+void keyswitch_group_exec(const hx_keyswitch_group* g, hipStream_t st) {
+    if (!g) raise(HX_ERR_INVALID_ARGUMENT, "group pointer must not be null.");
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != g->device) raise(HX_ERR_INVALID_ARGUMENT, "group belongs to another device");
+    for (const GroupLaunch& L : g->launches) {
+        const char* base = (const char*)g->mem;
+        const int64_t* starts = (const int64_t*)(base + L.starts_off);
+        if (L.phase == 2) {
+            const InnerArgs* list = (const InnerArgs*)(base + L.args_off);
+            const dim3 grid((unsigned)L.blocks), block((unsigned)L.threads);
+            if (L.H == 8) hipLaunchKernelGGL(k_ks_inner_multi<8>, grid, block, L.lds, st, list, starts, L.count);
+            else if (L.H == 7) hipLaunchKernelGGL(k_ks_inner_multi<7>, grid, block, L.lds, st, list, starts, L.count);
+            else hipLaunchKernelGGL(k_ks_inner_multi<6>, grid, block, L.lds, st, list, starts, L.count);
+            check_hip(hipGetLastError(), "grouped keyswitch inner launch");
+        } else {
+            launch_pass_multi(L.H, L.phase == 3, (const PassArgs*)(base + L.args_off), starts, L.count, L.blocks, st);
+        }
+    }
+}
+
+void keyswitch_group_destroy(hx_keyswitch_group* g) {
+    if (!g) return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(g->mem);
+    for (void* w : g->workspaces) (void)hipFree(w);
+    delete g;
 }
 
 }  // namespace hx
@@ -604,6 +919,28 @@ int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const 
     HX_API_BEGIN
     if (key_limbs < 1 || key_count < 1) raise(HX_ERR_INVALID_ARGUMENT, "key_limbs and key_count must be positive");
     keyswitch_exec(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_group_create(hx_keyswitch_group** out, const hx_keyswitch_plan* const* plans, const hx_view* xs,
+                              const void* const* keys, const size_t* key_bytes, const hx_view* outs, int count,
+                              hx_stream stream) {
+    HX_API_BEGIN
+    if (!out) raise(HX_ERR_INVALID_ARGUMENT, "out pointer must not be null.");
+    *out = nullptr;
+    *out = keyswitch_group_create(plans, xs, keys, key_bytes, outs, count, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_group_exec(const hx_keyswitch_group* group, hx_stream stream) {
+    HX_API_BEGIN
+    keyswitch_group_exec(group, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_group_destroy(hx_keyswitch_group* group) {
+    HX_API_BEGIN
+    keyswitch_group_destroy(group);
     HX_API_END
 }
 

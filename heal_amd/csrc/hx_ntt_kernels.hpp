@@ -245,8 +245,8 @@ struct TileShared {
     int32_t valid[64];
 };
 
-__device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared& T) {
-    int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared& T, int64_t blk, int64_t nblk) {
+    int64_t tile = xcd_tile(blk, nblk);
     const int64_t col_tile = tile % P.n_col_tiles;
     tile /= P.n_col_tiles;
     const int64_t bot_tile = tile % P.n_bot_tiles;
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs P) {
     const int WP = W + 1;
     const int S = P.S_cols * P.S_bot;
     auto slot = [WP](int s, int w) { return s * WP + w; };
-    tile_setup(P, S, T);
+    tile_setup(P, S, T, blockIdx.x, gridDim.x);
     tile_load(P, S, W, T, xs, slot);
 
     const int half_count = W >> 1;
@@ -599,8 +599,9 @@ __device__ __forceinline__ int r16_wc(int layout, int e) {
     return layout == 0 ? e << nb : ((e >> nb) << (2 * nb)) + (e & ((1 << nb) - 1));
 }
 
+// One workgroup of a radix-16 pass: tile `blk` of the pass's `nblk` tiles.
 template <int H, bool INV>
-__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
+__device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
     constexpr int W = 1 << H;
     constexpr int S = 32;               // sub-problems per workgroup
     constexpr int SP = W + 1;           // LDS row length (odd)
@@ -608,9 +609,9 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
     constexpr int L_OUT = 1 - L_IN;
     extern __shared__ uint64_t xs[];    // [S][SP]
     __shared__ TileShared T;
-    tile_setup(P, S, T);
+    tile_setup(P, S, T, blk, nblk);
     // the bits above the window are the same for the whole tile
-    const int64_t top = (xcd_tile(blockIdx.x, gridDim.x) / P.n_col_tiles) / P.n_bot_tiles;
+    const int64_t top = (xcd_tile(blk, nblk) / P.n_col_tiles) / P.n_bot_tiles;
 
     const int s = threadIdx.x % S, lane = threadIdx.x / S;
     const bool valid = T.valid[s];
@@ -748,6 +749,31 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
     }
 }
 
+template <int H, bool INV>
+__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
+    r16_pass<H, INV>(P, blockIdx.x, gridDim.x);
+}
+
+// Index of the instance whose tile range [starts[i], starts[i + 1]) holds blk.
+__device__ __forceinline__ int instance_of(const int64_t* starts, int count, int64_t blk) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Several passes of one window size in one launch: workgroup b runs tile
+// b - starts[i] of pass list[i] (starts has count + 1 entries).
+template <int H, bool INV>
+__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16_multi(const PassArgs* list, const int64_t* starts,
+                                                                       int count) {
+    const int i = instance_of(starts, count, blockIdx.x);
+    r16_pass<H, INV>(list[i], (int64_t)blockIdx.x - starts[i], starts[i + 1] - starts[i]);
+}
+
 // dst[u][col] = src[map[u]][col] (forward gather for non-permutation perms)
 struct GatherArgs {
     ColAddr src;
@@ -832,14 +858,43 @@ constexpr int kR16Tile = 32;  // sub-problems per workgroup of the radix-16 pass
 
 // Launches one pass.  P.S_cols / P.S_bot give the tile (sub-problems per
 // workgroup, see configure_tile); a radix-16 sub-problem takes 2^(H-4) lanes.
-void launch_pass(PassArgs& P, hipStream_t st) {
-    const int W = 1 << P.H;
-    const int S = P.S_cols * P.S_bot;
+// Fills P.n_bot_tiles / P.n_col_tiles and returns the pass's workgroup count.
+int64_t pass_blocks(PassArgs& P) {
     const int64_t ntop = int64_t(1) << (P.logm - P.b0 - P.H);
     P.n_bot_tiles = (P.bot_count + P.S_bot - 1) / P.S_bot;
     P.n_col_tiles = (P.ncols + P.S_cols - 1) / P.S_cols;
     const int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
     if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
+    return blocks;
+}
+
+// Radix-16 passes of one window size H (and direction) as one launch over
+// device arrays: list[count] of prepared PassArgs, starts[count + 1] tile offsets.
+void launch_pass_multi(int H, bool inverse, const PassArgs* list, const int64_t* starts, int count, int64_t blocks,
+                       hipStream_t st) {
+    if (!r16_window(H)) raise(HX_ERR_RUNTIME, "internal: grouped passes need radix-16 windows");
+    if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "too many tiles for one launch");
+    const int W = 1 << H;
+    const size_t lds = (size_t)kR16Tile * (W + 1) * 8;
+    const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
+#define HX_R16M(HH)                                                                                              \
+    if (inverse) hipLaunchKernelGGL((k_ntt_pass_r16_multi<HH, true>), grid, block, lds, st, list, starts, count);  \
+    else hipLaunchKernelGGL((k_ntt_pass_r16_multi<HH, false>), grid, block, lds, st, list, starts, count);
+    if (H == 8) {
+        HX_R16M(8)
+    } else if (H == 7) {
+        HX_R16M(7)
+    } else {
+        HX_R16M(6)
+    }
+#undef HX_R16M
+    check_hip(hipGetLastError(), "grouped ntt pass launch");
+}
+
+void launch_pass(PassArgs& P, hipStream_t st) {
+    const int W = 1 << P.H;
+    const int S = P.S_cols * P.S_bot;
+    const int64_t blocks = pass_blocks(P);
     if (r16_window(P.H)) {
         if (S != kR16Tile) raise(HX_ERR_RUNTIME, "internal: radix-16 tile size");
         const size_t lds = (size_t)kR16Tile * (W + 1) * 8;

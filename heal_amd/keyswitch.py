@@ -124,3 +124,111 @@ class KeySwitchBatch:
         intt = 4 * P_
         perm = 4 * P_
         return gd + ntt + mm + ams + intt + perm
+
+
+class UnevenKeySwitch:
+    """`count` independent key-switches of mixed size: N uniform in `ns`, L
+    uniform in [limbs[0], limbs[1]] (seeded); the primes of an instance are the
+    first L of the 24-prime key-switch list (q = 1 mod 2^17, so 1 mod 2N for
+    every N here); x and the evaluation key uniform mod q.  Instances of the
+    same (N, L) share plans.
+
+    run_plain() runs them one key-switch call at a time; group() builds the
+    grouped form (one launch per phase and window size for all of them)."""
+
+    def __init__(self, count: int = 64, ns=(1 << 13, 1 << 14, 1 << 15, 1 << 16), limbs=(4, 24), dnum: int = 3,
+                 base_bits: int = 60, seed: int = 2, device: str = "cuda"):
+        self.device = torch.device(device)
+        self.dnum, self.w = dnum, base_bits
+        g = torch.Generator().manual_seed(seed)
+        self.shapes = [(ns[int(torch.randint(0, len(ns), (1,), generator=g))],
+                        int(torch.randint(limbs[0], limbs[1] + 1, (1,), generator=g))) for _ in range(count)]
+        self.primes = P.keyswitch_primes(max(L for _, L in self.shapes))
+        self.params = {n: P.NttParams(n, self.primes, device=self.device) for n in sorted({n for n, _ in self.shapes})}
+        self.plans = {}
+        gd = torch.Generator(device=self.device)
+        gd.manual_seed(seed)
+        q = torch.tensor(self.primes, dtype=torch.int64, device=self.device)
+        self.x, self.evk, self.keys, self.outs, self.members = [], [], [], [], []
+        for n, L in self.shapes:
+            plan = self.plan(n, L)
+            x = torch.randint(0, 1 << 62, (1, n, L), generator=gd, device=self.device) % q[:L]
+            evk = torch.randint(0, 1 << 62, (dnum, 1, 2, n, L), generator=gd, device=self.device) % q[:L]
+            self.x.append(x)
+            self.evk.append(evk)
+            self.keys.append(plan.prepare_key(evk))
+            self.outs.append(torch.empty((1, 2, n, L), dtype=torch.int64, device=self.device))
+            self.members.append(plan)
+
+    def plan(self, n: int, L: int):
+        if (n, L) not in self.plans:
+            prm = self.params[n]
+            fwd = capi.NttPlan(prm.p[:L], prm.perm, prm.twiddles[:L])
+            inv = capi.NttPlan(prm.p[:L], prm.perm, prm.inv_twiddles[:L], prm.m_inv[:L], inverse=True)
+            perms = P.galois_perms(n, 5, L, device=self.device)
+            self.plans[(n, L)] = (capi.KeySwitchPlan(fwd, inv, perms, self.dnum, self.w), fwd, inv, perms)
+        return self.plans[(n, L)][0]
+
+    def run_plain(self, outs=None):
+        outs = self.outs if outs is None else outs
+        for plan, x, key, out in zip(self.members, self.x, self.keys, outs):
+            plan(x, key, out)
+        return outs
+
+    def group(self, outs=None):
+        return capi.KeySwitchGroup(self.members, self.x, self.keys, self.outs if outs is None else outs)
+
+    def algorithmic_bytes(self) -> int:
+        """Read x and the key, write the output: 8 N L (1 + 2 dnum + 2) per instance."""
+        return sum(8 * n * L * (1 + 2 * self.dnum + 2) for n, L in self.shapes)
+
+
+def _uneven_main():
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description="uneven batch of key-switches: plain calls vs one grouped launch")
+    ap.add_argument("--count", type=int, default=64)
+    ap.add_argument("--reps", type=int, default=20)
+    args = ap.parse_args()
+    t = time.perf_counter()
+    u = UnevenKeySwitch(count=args.count)
+    setup = time.perf_counter() - t
+    grp = u.group([torch.empty_like(o) for o in u.outs])
+    u.run_plain()
+    grp()
+    torch.cuda.synchronize()
+    equal = all(torch.equal(a, b) for a, b in zip(u.outs, grp._keep[3]))
+    graph = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        grp()
+    torch.cuda.current_stream().wait_stream(s)
+    with torch.cuda.graph(graph):
+        grp()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1000.0 / args.reps
+
+    t_plain, t_group, t_graph = timed(u.run_plain), timed(grp), timed(graph.replay)
+    n = args.count
+    print(json.dumps({
+        "metric": f"key-switches/s, {n} instances of mixed N in 2^13..2^16 and L in [4, 24] (dnum 3, w 60)",
+        "plain_ks_per_s": round(n / t_plain, 1), "grouped_ks_per_s": round(n / t_group, 1),
+        "grouped_graph_ks_per_s": round(n / t_graph, 1), "grouped_equal_plain": equal,
+        "algorithmic_GBps_graph": round(u.algorithmic_bytes() / t_graph / 1e9, 1),
+        "shapes": sorted(u.shapes), "setup_s": round(setup, 2)}), flush=True)
+
+
+if __name__ == "__main__":
+    _uneven_main()

@@ -183,7 +183,12 @@ int hx_permute_ex(const hx_view* a, const hx_view* perms, const hx_view* result,
  * 2^12 <= m <= 2^16.  The key is laid out once for the device by
  * hx_keyswitch_prepare_key into a caller-allocated buffer of
  * hx_keyswitch_key_bytes(plan, B) bytes; exec then runs three kernels and never
- * synchronises.  g_decomposition overflow is counted as by hx_g_decomposition. */
+ * synchronises.  g_decomposition overflow is counted as by hx_g_decomposition.
+ * Plan creation times a few launch shapes on scratch data (the split of log2 m
+ * between the high and the low window, chosen once per (m, dnum) in a process,
+ * and the limbs per inner workgroup, per (m, k, dnum)); every choice gives the
+ * same result.  A key prepared by one plan serves every plan of the same m,
+ * dnum, moduli prefix and NTT permutation in the process. */
 typedef struct hx_keyswitch_plan hx_keyswitch_plan;
 int hx_keyswitch_plan_create(hx_keyswitch_plan** out, const hx_ntt_plan* fwd, const hx_ntt_plan* inv,
                              const hx_view* perms, uint64_t dnum, uint64_t base_bits, hx_stream stream);
@@ -204,6 +209,20 @@ int hx_keyswitch_plan_destroy(hx_keyswitch_plan* plan);
 int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                          int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
                          hx_stream stream);
+/* Key-switches of mixed ring degree and limb count run together: the members
+ * (plan i on x[i] with key[i], into out[i], exactly as hx_keyswitch_exec) are
+ * fixed at creation and hx_keyswitch_group_exec runs all of them with one
+ * launch per phase (K1, K2, K3) and window size, reading per-member launch
+ * arguments from device memory.  Every member gets its own workspace, so
+ * members may share a plan.  Results equal hx_keyswitch_exec of each member.
+ * No counterpart in the reference. */
+typedef struct hx_keyswitch_group hx_keyswitch_group;
+int hx_keyswitch_group_create(hx_keyswitch_group** out, const hx_keyswitch_plan* const* plans, const hx_view* xs,
+                              const void* const* keys, const size_t* key_bytes, const hx_view* outs, int count,
+                              hx_stream stream);
+int hx_keyswitch_group_exec(const hx_keyswitch_group* group, hx_stream stream);
+int hx_keyswitch_group_destroy(hx_keyswitch_group* group);
+
 /* row_map[perms[n]] = n for one permutation perms [m] (or [1, m]) of [0, m);
  * validates perms (synchronises the stream). */
 int hx_keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int32_t* row_map, hx_stream stream);
