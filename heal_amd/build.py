@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libheal_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HEAL_AMD_ARCH", "gfx950")
 
-SOURCES = ["hx_runtime.hip", "hx_elementwise.hip", "hx_ntt.hip", "hx_keyswitch.hip"]
+SOURCES = ["hx_runtime.hip", "hx_elementwise.hip", "hx_ntt.hip", "hx_keyswitch.hip", "hx_ntt_mfma.hip"]
 
 HIP_FLAGS = [
     "-O3",

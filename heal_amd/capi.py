@@ -95,6 +95,10 @@ _PROTOS = {
                                   ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), VP, ctypes.c_int,
                                   STREAM],
     "hx_keyswitch_group_exec": [ctypes.c_void_p, STREAM],
+    "hx_ntt_mfma_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p, STREAM],
+    "hx_ntt_mfma_exec": [ctypes.c_void_p, VP, VP, STREAM],
+    "hx_ntt_mfma_plan_destroy": [ctypes.c_void_p],
     "hx_keyswitch_group_destroy": [ctypes.c_void_p],
 }
 

@@ -1,0 +1,325 @@
+// hx_ntt_mfma.hip — exact negacyclic NTT of length 65536 for primes q < 2^28 as
+// a four-step transform whose 256-point sub-transforms are INT8 matrix-core
+// products (v_mfma_i32_16x16x64_i8 on gfx950).
+//
+// The transform is the reference's NTT with the standard tables
+// (example_impl/ntt_impl.cpp:69-116 with twiddles[t, j] = psi^bitrev(j) and
+// perm = bitrev): out[u] = sum_i a[i] psi^((2u+1) i) mod q.  With
+// omega = psi^2, i = i1 + 256 i2 and u = u2 + 256 u1:
+//   step 1  C[u2, i1]  = sum_i2 W2[u2, i2] a[256 i2 + i1],  W2 = omega^(256 u2 i2) psi^(256 i2)
+//           C'[u2, i1] = C[u2, i1] * T[u2, i1] mod q,       T  = psi^i1 omega^(u2 i1)
+//           stored transposed: mid[i1][u2] = C'[u2, i1]
+//   step 2  out[256 u1 + u2] = sum_i1 W1[u1, i1] mid[i1][u2], W1 = omega^(256 u1 i1)
+// Both steps are 256 x 256 x 256 products per column with a constant left
+// matrix.  Operands (< 2^28) are split into four 7-bit int8 planes; the 16
+// plane products accumulate in int32 grouped by s = l + k (each group sums at
+// most 4 * 256 * 127^2 < 2^24), and a value is recombined as
+// sum_s acc_s * (2^(7 s) mod q) < 2^55, then reduced by Barrett with
+// mu = floor((2^64 - 1) / q): for v < 2^56 the estimate floor(v mu / 2^64) is
+// floor(v / q) or one less, so one conditional subtraction finishes.  Every
+// step is exact, so the output equals the radix-2 network's bit for bit.
+//
+// MFMA operand layout (checked on the device): lane l supplies row l mod 16 of
+// A and column l mod 16 of B with 16 int8 each, k = 16 (l / 16) + i within a
+// 64-wide k chunk (A and B agree); the 16 x 16 int32 result gives lane l rows
+// 4 (l / 16) + j, j < 4, of column l mod 16.
+#include <cstring>
+#include <new>
+
+#include "hx_internal.hpp"
+
+struct hx_ntt_mfma_plan {
+    int device;
+    int64_t k;      // limbs
+    void* mem;      // tables (below)
+    uint4* w2;      // [k][4 planes][16 u-tiles][4 k-chunks][64 lanes]
+    uint4* w1;      // same for W1
+    uint32_t* tmat; // [k][256 u2][256 i1]
+    uint64_t* red;  // [k][10]: q, mu, R_0..R_6, pad
+    mutable void* ws;
+    mutable size_t ws_bytes;
+};
+
+namespace hx {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kRows = 256;
+constexpr int kBlockCols = 32;            // columns of the right operand per workgroup
+constexpr int kColBlocks = kRows / kBlockCols;
+constexpr int kThreads4 = 512;            // 8 waves: 32 rows of the product each
+
+struct Ntt4Args {
+    const int64_t* a;      // input element (p, i, t) at a + p*a_sp + t*a_st + i*a_si
+    int64_t a_sp, a_st, a_si;
+    int64_t* out;
+    int64_t o_sp, o_st, o_si;
+    uint32_t* mid;         // [P*K][256 i1][256 u2]
+    const uint4* w;        // the step's left matrix planes
+    const uint32_t* tmat;
+    const uint64_t* red;
+    int64_t P, K;
+};
+
+__device__ __forceinline__ uint64_t barrett(uint64_t v, uint64_t q, uint64_t mu) {
+    const uint64_t qt = __umul64hi(v, mu);
+    uint64_t r = v - qt * q;
+    return r >= q ? r - q : r;
+}
+
+// residue in [0, q) of a signed input value (inputs outside [0, q) are rare:
+// a plain 64-bit remainder)
+__device__ __forceinline__ uint64_t reduce_in(int64_t x, uint64_t q) {
+    if (x >= 0 && (uint64_t)x < q) return (uint64_t)x;
+    const uint64_t ax = x < 0 ? (uint64_t)0 - (uint64_t)x : (uint64_t)x;
+    const uint64_t r = ax % q;
+    return x < 0 ? (r ? q - r : 0) : r;
+}
+
+template <int STEP>
+__global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
+    __shared__ uint4 bs[2 * 4 * 4 * 64];  // right operand planes: [c-tile][plane][k-chunk][lane]
+    // workgroups of one limb are adjacent (they share the left matrix in L2)
+    // (blocks are dispatched to the 8 XCDs round-robin: give each XCD a contiguous range)
+    const int64_t nblk = A.P * A.K * kColBlocks;
+    const int64_t xcd = (int64_t)blockIdx.x % 8, rem = nblk % 8;
+    const int64_t blk = xcd * (nblk / 8) + (xcd < rem ? xcd : rem) + (int64_t)blockIdx.x / 8;
+    const int64_t t = blk / (A.P * kColBlocks);
+    const int64_t p = (blk / kColBlocks) % A.P;
+    const int cb = (int)(blk % kColBlocks);
+    const int64_t col = p * A.K + t;
+    const uint64_t* rd = A.red + t * 10;
+    const uint64_t q = rd[0], mu = rd[1];
+    const int tid = threadIdx.x;
+
+    // ---- right operand: 256 k rows x 32 columns, split into planes
+    {
+        const int c = tid % kBlockCols, run = tid / kBlockCols;  // 16 runs of 16 k
+        const int gc = cb * kBlockCols + c;
+        uint32_t v[16];
+        if (STEP == 1) {
+            const int64_t* base = A.a + p * A.a_sp + t * A.a_st;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int64_t k = 16 * run + i;  // i2
+                v[i] = (uint32_t)reduce_in(base[(k * kRows + gc) * A.a_si], q);
+            }
+        } else {
+            const uint32_t* base = A.mid + col * (kRows * kRows);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = base[(16 * run + i) * kRows + gc];  // mid[i1][u2]
+        }
+        const int ct = c / 16, lane = (run % 4) * 16 + c % 16, kq = run / 4;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            uint32_t w4[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) word |= ((v[4 * b + j] >> (7 * l)) & 127u) << (8 * j);
+                w4[b] = word;
+            }
+            bs[((ct * 4 + l) * 4 + kq) * 64 + lane] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
+    }
+    __syncthreads();
+
+    // ---- 32 rows x 32 columns per wave: 2 x 2 tiles, 7 accumulators each
+    const int wave = tid / 64, lane = tid % 64;
+    v4i acc[2][2][7];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int s = 0; s < 7; ++s) acc[a][b][s] = v4i{0, 0, 0, 0};
+    const uint4* wl = A.w + t * (4 * 16 * 4 * 64);
+#pragma unroll
+    for (int kq = 0; kq < 4; ++kq) {
+        __builtin_amdgcn_sched_barrier(0);  // one k chunk of operands live at a time
+        v4i bm[2][4];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const uint4 x = bs[((ct * 4 + l) * 4 + kq) * 64 + lane];
+                bm[ct][l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+            }
+#pragma unroll
+        for (int ut = 0; ut < 2; ++ut) {
+            v4i am[4];
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const uint4 x = wl[((l * 16 + (2 * wave + ut)) * 4 + kq) * 64 + lane];
+                am[l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+            }
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int la = 0; la < 4; ++la)
+#pragma unroll
+                    for (int lb = 0; lb < 4; ++lb)
+                        acc[ut][ct][la + lb] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(am[la], bm[ct][lb], acc[ut][ct][la + lb], 0, 0, 0);
+        }
+    }
+
+    __builtin_amdgcn_sched_barrier(0);
+    // ---- recombine, reduce, and write
+    uint64_t R[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) R[s] = rd[2 + s];
+#pragma unroll
+    for (int ut = 0; ut < 2; ++ut)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int row0 = (2 * wave + ut) * 16 + 4 * (lane / 16);
+            const int gc = cb * kBlockCols + ct * 16 + lane % 16;
+            uint32_t r4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint64_t v = 0;
+#pragma unroll
+                for (int s = 0; s < 7; ++s) v += (uint64_t)(uint32_t)acc[ut][ct][s][j] * R[s];
+                v = barrett(v, q, mu);
+                if (STEP == 1) {
+                    const uint64_t tw = A.tmat[(t * kRows + row0 + j) * kRows + gc];  // T[u2][i1]
+                    v = barrett(v * tw, q, mu);
+                }
+                r4[j] = (uint32_t)v;
+            }
+            if (STEP == 1) {  // mid[i1 = gc][u2 = row0 .. row0 + 3]
+                *(uint4*)(A.mid + col * (kRows * kRows) + (int64_t)gc * kRows + row0) =
+                    make_uint4(r4[0], r4[1], r4[2], r4[3]);
+            } else {          // out[256 u1 + u2], u1 = row, u2 = gc
+                int64_t* ob = A.out + p * A.o_sp + t * A.o_st;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) ob[((int64_t)(row0 + j) * kRows + gc) * A.o_si] = (int64_t)r4[j];
+            }
+        }
+}
+
+}  // namespace
+
+static void require_ntt4_view(const hx_view* v, const char* name, int64_t k) {
+    check_view(v, name);
+    if (v->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must be int64");
+    if (v->ndim != 4 || v->dims[1] != 65536 || v->dims[2] != 1 || v->dims[3] != k)
+        raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must have shape [l, 65536, 1, k]");
+}
+
+hx_ntt_mfma_plan* ntt_mfma_plan_create(int64_t k, const void* w2, const void* w1, const void* tmat, const void* red,
+                                       hipStream_t st) {
+    if (k < 1) raise(HX_ERR_INVALID_ARGUMENT, "k must be positive");
+    if (!w2 || !w1 || !tmat || !red) raise(HX_ERR_INVALID_ARGUMENT, "table pointers must not be null.");
+    const size_t wb = (size_t)k * 4 * 16 * 4 * 64 * 16, tb = (size_t)k * kRows * kRows * 4, rb = (size_t)k * 10 * 8;
+    auto* plan = new hx_ntt_mfma_plan;
+    std::memset(plan, 0, sizeof *plan);
+    plan->k = k;
+    check_hip(hipGetDevice(&plan->device), "hipGetDevice");
+    hipError_t e = hipMalloc(&plan->mem, 2 * wb + tb + rb);
+    if (e != hipSuccess) {
+        delete plan;
+        check_hip(e, "hipMalloc(mfma ntt plan)");
+    }
+    char* m = (char*)plan->mem;
+    plan->w2 = (uint4*)m;
+    plan->w1 = (uint4*)(m + wb);
+    plan->tmat = (uint32_t*)(m + 2 * wb);
+    plan->red = (uint64_t*)(m + 2 * wb + tb);
+    try {
+        check_hip(hipMemcpyAsync(plan->w2, w2, wb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        check_hip(hipMemcpyAsync(plan->w1, w1, wb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        check_hip(hipMemcpyAsync(plan->tmat, tmat, tb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        check_hip(hipMemcpyAsync(plan->red, red, rb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipFree(plan->mem);
+        delete plan;
+        throw;
+    }
+    return plan;
+}
+
+void ntt_mfma_exec(const hx_ntt_mfma_plan* plan, const hx_view* a, const hx_view* out, hipStream_t st) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    require_ntt4_view(a, "a", plan->k);
+    require_ntt4_view(out, "result", plan->k);
+    if (a->dims[0] != out->dims[0]) raise(HX_ERR_INVALID_ARGUMENT, "Output tensor must have the same shape as input tensor.");
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != plan->device) raise(HX_ERR_INVALID_ARGUMENT, "plan belongs to another device");
+    const int64_t P = a->dims[0], K = plan->k;
+    if (P == 0) return;
+    const size_t bytes = (size_t)P * K * kRows * kRows * 4;
+    if (plan->ws_bytes < bytes) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone)
+            raise(HX_ERR_RUNTIME, "the plan's workspace must be sized by one execution before graph capture");
+        if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync");
+        plan->ws = nullptr;
+        plan->ws_bytes = 0;
+        check_hip(hipMallocAsync(&plan->ws, bytes, st), "hipMallocAsync(mfma ntt workspace)");
+        plan->ws_bytes = bytes;
+    }
+    Ntt4Args A;
+    std::memset(&A, 0, sizeof A);
+    A.a = (const int64_t*)a->data;
+    A.a_sp = a->strides[0], A.a_st = a->strides[3], A.a_si = a->strides[1];
+    A.out = (int64_t*)out->data;
+    A.o_sp = out->strides[0], A.o_st = out->strides[3], A.o_si = out->strides[1];
+    A.mid = (uint32_t*)plan->ws;
+    A.tmat = plan->tmat;
+    A.red = plan->red;
+    A.P = P;
+    A.K = K;
+    const int64_t blocks = P * K * kColBlocks;
+    if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "too many columns for one launch");
+    if (views_overlap(a, out) && !(a->data == out->data)) raise(HX_ERR_INVALID_ARGUMENT, "a and result overlap");
+    A.w = plan->w2;
+    hipLaunchKernelGGL(k_ntt4<1>, dim3((unsigned)blocks), dim3(kThreads4), 0, st, A);
+    check_hip(hipGetLastError(), "mfma ntt step 1 launch");
+    A.w = plan->w1;
+    hipLaunchKernelGGL(k_ntt4<2>, dim3((unsigned)blocks), dim3(kThreads4), 0, st, A);
+    check_hip(hipGetLastError(), "mfma ntt step 2 launch");
+}
+
+void ntt_mfma_plan_destroy(hx_ntt_mfma_plan* plan) {
+    if (!plan) return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(plan->mem);
+    if (plan->ws) (void)hipFree(plan->ws);
+    delete plan;
+}
+
+}  // namespace hx
+
+using namespace hx;
+
+extern "C" {
+
+int hx_ntt_mfma_plan_create(hx_ntt_mfma_plan** out, int64_t k, const void* w2, const void* w1, const void* tmat,
+                            const void* red, hx_stream stream) {
+    HX_API_BEGIN
+    if (!out) raise(HX_ERR_INVALID_ARGUMENT, "out pointer must not be null.");
+    *out = nullptr;
+    *out = ntt_mfma_plan_create(k, w2, w1, tmat, red, as_stream(stream));
+    HX_API_END
+}
+
+int hx_ntt_mfma_exec(const hx_ntt_mfma_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    ntt_mfma_exec(plan, a, result, as_stream(stream));
+    HX_API_END
+}
+
+int hx_ntt_mfma_plan_destroy(hx_ntt_mfma_plan* plan) {
+    HX_API_BEGIN
+    ntt_mfma_plan_destroy(plan);
+    HX_API_END
+}
+
+}  // extern "C"

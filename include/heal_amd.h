@@ -209,6 +209,24 @@ int hx_keyswitch_plan_destroy(hx_keyswitch_plan* plan);
 int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                          int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
                          hx_stream stream);
+/* Exact negacyclic NTT of length 65536 for primes q < 2^28 on the INT8 matrix
+ * cores (four-step: 256-point sub-transforms as 256 x 256 x 256 products of
+ * 7-bit int8 planes with int32 accumulation, recombined and reduced mod q).
+ * Equal to hx_ntt with twiddles[t, j] = psi_t^bitrev(j) and perm = bitrev.
+ * The plan takes host tables (heal_amd/ntt_mfma.py builds them from q and psi):
+ *   w2, w1  [k][4][16][4][64] x 16 bytes: the step matrices
+ *           W2[u2, i2] = omega^(256 u2 i2) psi^(256 i2), W1[u1, i1] = omega^(256 u1 i1)
+ *           (omega = psi^2), 7-bit planes in MFMA operand order;
+ *   tmat    [k][256][256] uint32: T[u2, i1] = psi^i1 omega^(u2 i1);
+ *   red     [k][10] uint64: q, floor((2^64 - 1) / q), 2^(7 s) mod q (s < 7), 0.
+ * a, result: int64 [l, 65536, 1, k] views (limb-major layouts read columns
+ * contiguously).  No counterpart in the reference. */
+typedef struct hx_ntt_mfma_plan hx_ntt_mfma_plan;
+int hx_ntt_mfma_plan_create(hx_ntt_mfma_plan** out, int64_t k, const void* w2, const void* w1, const void* tmat,
+                            const void* red, hx_stream stream);
+int hx_ntt_mfma_exec(const hx_ntt_mfma_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream);
+int hx_ntt_mfma_plan_destroy(hx_ntt_mfma_plan* plan);
+
 /* Key-switches of mixed ring degree and limb count run together: the members
  * (plan i on x[i] with key[i], into out[i], exactly as hx_keyswitch_exec) are
  * fixed at creation and hx_keyswitch_group_exec runs all of them with one

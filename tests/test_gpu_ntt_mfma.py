@@ -1,0 +1,76 @@
+"""The matrix-core NTT (N = 65536, primes < 2^28): bit-exact against the ALU
+NTT on the same inputs (limb-major and limb-last layouts, inputs outside
+[0, q) included) and against the oracle's radix-2 network on whole columns."""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from heal_cases import OracleBackend
+
+pytestmark = pytest.mark.gpu
+
+N = 65536
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    from heal_amd import capi
+    from heal_amd.ntt_mfma import NttMfmaPlan, small_primes
+    from heal_amd.params import NttParams
+
+    primes = small_primes(4)
+    prm = NttParams(N, primes, device="cuda")
+    return primes, prm, NttMfmaPlan(primes, prm.psi), capi.NttPlan(prm.p, prm.perm, prm.twiddles)
+
+
+def test_matches_alu_ntt_limb_major(setup):
+    primes, prm, mfma, alu = setup
+    print("[mfma] limb-major", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    q = prm.p.view(1, -1, 1)
+    data = torch.randint(0, 1 << 62, (3, 4, N), generator=g, device="cuda") % q
+    data[0, 0, :5] = torch.tensor([-1, -(1 << 40), 1 << 62, primes[0], primes[0] + 3], device="cuda")
+    a = data.permute(0, 2, 1).unsqueeze(2)
+    o1 = torch.full_like(data, -5).permute(0, 2, 1).unsqueeze(2)
+    o2 = torch.full_like(data, -6).permute(0, 2, 1).unsqueeze(2)
+    mfma(a, o1)
+    alu(a, o2)
+    assert torch.equal(o1, o2)
+
+
+def test_matches_alu_ntt_limb_last_and_in_place(setup):
+    primes, prm, mfma, alu = setup
+    print("[mfma] limb-last", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    a = torch.randint(0, 1 << 62, (2, N, 1, 4), generator=g, device="cuda") % prm.p
+    exp = torch.empty_like(a)
+    alu(a, exp)
+    mfma(a, a)  # in place
+    assert torch.equal(a, exp)
+
+
+def test_matches_oracle_columns(setup):
+    primes, prm, mfma, alu = setup
+    print("[mfma] oracle", flush=True)
+    o = OracleBackend()
+    g = torch.Generator().manual_seed(3)
+    a = torch.randint(0, 1 << 62, (1, N, 1, 4), generator=g) % prm.p.cpu()
+    out = torch.empty_like(a).cuda()
+    mfma(a.cuda(), out)
+    for t in (0, 3):
+        exp = torch.zeros(1, N, 1, 1, dtype=torch.int64)
+        assert o.ntt(a[..., t:t + 1].contiguous(), prm.p[t:t + 1].cpu(), prm.perm.cpu(),
+                     prm.twiddles[t:t + 1].cpu(), exp) is None
+        assert torch.equal(out[..., t:t + 1].cpu(), exp)
+
+
+def test_rejects_large_primes(gpu):
+    from heal_amd.ntt_mfma import tables
+    from heal_amd.params import keyswitch_primes, primitive_root_2n
+
+    q = keyswitch_primes(1)[0]
+    with pytest.raises(ValueError):
+        tables([q], [primitive_root_2n(q, N)])
