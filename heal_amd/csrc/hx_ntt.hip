@@ -223,7 +223,10 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         tmp = plan->ws;
     }
     void* tmp2 = (need_tmp && plan->perm_kind >= PERM_GATHER_FWD) ? (char*)tmp + (size_t)m * ncols * 8 : nullptr;
-    const ColAddr T1 = need_tmp ? tmp_addr(tmp, ncols) : ColAddr{};
+    // the intermediate follows the data: columns innermost for limb-last
+    // layouts, rows innermost for limb-major ones (coalesced either way)
+    auto tmp_at = [&](void* ptr) { return rows_contiguous ? tmp_addr_rows(ptr, m) : tmp_addr(ptr, ncols); };
+    const ColAddr T1 = need_tmp ? tmp_at(tmp) : ColAddr{};
 
     // inverse with a non-permutation perm: stage the scattered input first
     ColAddr in_src = A;
@@ -234,7 +237,7 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         PassArgs G = base;
         G.src = A;
         G.alt = R;
-        G.dst = tmp_addr(tmp2, ncols);
+        G.dst = tmp_at(tmp2);
         G.src_map = plan->rowmap;
         G.src_map_alt = 1;
         G.H = 0;
@@ -246,7 +249,7 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         G.reduce_in = 0;
         G.scale_out = 0;
         launch_pass(G, st);
-        in_src = tmp_addr(tmp2, ncols);
+        in_src = tmp_at(tmp2);
     }
 
     int done_bits = 0;
@@ -265,21 +268,25 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         const bool first = ip == 0, last = ip == npass - 1;
         P.src = first ? in_src : T1;
         P.dst = last ? R : T1;
-        if (last && !plan->inverse && plan->perm_kind == PERM_GATHER_FWD) P.dst = tmp_addr(tmp2, ncols);
+        if (last && !plan->inverse && plan->perm_kind == PERM_GATHER_FWD) P.dst = tmp_at(tmp2);
         P.reduce_in = first && (plan->inverse || m > 1);
         P.scale_out = last && plan->inverse;
         if (first && plan->inverse && plan->perm_kind == PERM_MAP) P.src_map = plan->rowmap;
         if (last && !plan->inverse && plan->perm_kind == PERM_MAP) P.dst_map = plan->rowmap;
         // radix-16 passes hand lazy residues to a following radix-16 pass
         P.lazy_out = !last && r16_window(H) && r16_window(win[ip + 1]);
-        configure_tile(P, !(rows_contiguous && P.bot_count >= 8), npass);
+        const bool col_tiles = !(rows_contiguous && P.bot_count >= 8);
+        configure_tile(P, col_tiles, npass);
+        // a tile of whole columns whose rows are contiguous: lanes of a sub-problem
+        // take adjacent rows (coalesced), not adjacent columns
+        P.lane_major = col_tiles && rows_contiguous;
         launch_pass(P, st);
         done_bits += H;
     }
 
     if (!plan->inverse && plan->perm_kind == PERM_GATHER_FWD) {
         GatherArgs G;
-        G.src = tmp_addr(tmp2, ncols);
+        G.src = tmp_at(tmp2);
         G.dst = R;
         G.ncols = ncols;
         for (int i = 0; i < 3; ++i) G.cdim[i] = base.cdim[i];

@@ -194,6 +194,7 @@ struct PassArgs {
     int32_t reduce_in;
     int32_t scale_out;
     int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 4q) forward, [0, 2q) inverse)
+    int32_t lane_major;      // radix-16 passes: a sub-problem's lanes are adjacent threads (rows contiguous in memory)
     // digit source (src.kind == 3): column axis `gd_axis` is the digit index d,
     // the value loaded is digit d of x in base 2^gd_w; elements of digit 0 whose
     // value is not exhausted after gd_D digits are counted in gd_status
@@ -599,12 +600,24 @@ __device__ __forceinline__ int r16_wc(int layout, int e) {
     return layout == 0 ? e << nb : ((e >> nb) << (2 * nb)) + (e & ((1 << nb) - 1));
 }
 
+// LDS slot of window index w.  LM (lane-major thread order, for layouts whose
+// rows are contiguous): one pad slot every 16 and rows of W + W/16, so that 16
+// adjacent lanes of either layout and the sub-problems of a half-wave hit
+// different banks; otherwise odd rows of W + 1 (32 adjacent threads are 32
+// different sub-problems).
+template <int H, bool LM>
+struct R16Lds {
+    static constexpr int W = 1 << H;
+    static constexpr int SP = LM ? W + W / 16 : W + 1;
+    __device__ static constexpr int slot(int w) { return LM ? w + (w >> 4) : w; }
+};
+
 // One workgroup of a radix-16 pass: tile `blk` of the pass's `nblk` tiles.
-template <int H, bool INV>
+template <int H, bool INV, bool LM = false>
 __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
     constexpr int W = 1 << H;
     constexpr int S = 32;               // sub-problems per workgroup
-    constexpr int SP = W + 1;           // LDS row length (odd)
+    constexpr int SP = R16Lds<H, LM>::SP;
     constexpr int L_IN = INV ? 1 : 0;   // layout of the first stages
     constexpr int L_OUT = 1 - L_IN;
     extern __shared__ uint64_t xs[];    // [S][SP]
@@ -613,7 +626,9 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
     // the bits above the window are the same for the whole tile
     const int64_t top = (xcd_tile(blk, nblk) / P.n_col_tiles) / P.n_bot_tiles;
 
-    const int s = threadIdx.x % S, lane = threadIdx.x / S;
+    constexpr int TL = W / 16;
+    const int s = LM ? threadIdx.x / TL : threadIdx.x % S;
+    const int lane = LM ? threadIdx.x % TL : threadIdx.x / S;
     const bool valid = T.valid[s];
     const int limb = T.limb[s];
     const LimbInfo& L = P.limbs[limb];
@@ -708,10 +723,10 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
     if (valid) r16_stages_any<H, INV, L_IN>(x, lane, twrow, top, P.logm, P.stage0, L, mode);
     uint64_t* xs_s = xs + s * SP;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) xs_s[r16_w<H>(L_IN, e, lane)] = x[e];
+    for (int e = 0; e < 16; ++e) xs_s[R16Lds<H, LM>::slot(r16_w<H>(L_IN, e, lane))] = x[e];
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; ++e) x[e] = xs_s[r16_w<H>(L_OUT, e, lane)];
+    for (int e = 0; e < 16; ++e) x[e] = xs_s[R16Lds<H, LM>::slot(r16_w<H>(L_OUT, e, lane))];
     if (!valid) return;
     // the last pass of an inverse transform ends with the global stage logm - 1
     // (window bit H - 1, the last stage of layout A), where m^-1 is folded in
@@ -749,9 +764,9 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
     }
 }
 
-template <int H, bool INV>
+template <int H, bool INV, bool LM>
 __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
-    r16_pass<H, INV>(P, blockIdx.x, gridDim.x);
+    r16_pass<H, INV, LM>(P, blockIdx.x, gridDim.x);
 }
 
 // Index of the instance whose tile range [starts[i], starts[i + 1]) holds blk.
@@ -835,6 +850,18 @@ ColAddr tensor_addr(const hx_view* v, const int* order) {
     return c;
 }
 
+// intermediate with the rows of a column contiguous: column c at ptr + c * m
+ColAddr tmp_addr_rows(void* ptr, int64_t m) {
+    ColAddr c;
+    c.ptr = ptr;
+    c.row_stride = 1;
+    c.cstride[0] = m;
+    c.cstride[1] = 0;
+    c.cstride[2] = 0;
+    c.kind = 2;
+    return c;
+}
+
 ColAddr tmp_addr(void* ptr, int64_t ncols) {
     ColAddr c;
     c.ptr = ptr;
@@ -897,17 +924,30 @@ void launch_pass(PassArgs& P, hipStream_t st) {
     const int64_t blocks = pass_blocks(P);
     if (r16_window(P.H)) {
         if (S != kR16Tile) raise(HX_ERR_RUNTIME, "internal: radix-16 tile size");
-        const size_t lds = (size_t)kR16Tile * (W + 1) * 8;
+        const bool lm = P.lane_major != 0;
+        const size_t lds = (size_t)kR16Tile * (lm ? W + W / 16 : W + 1) * 8;
         const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
-#define HX_R16(HH)                                                                            \
-    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true>), grid, block, lds, st, P);   \
-    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false>), grid, block, lds, st, P);
+#define HX_R16(HH, LMV)                                                                              \
+    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true, LMV>), grid, block, lds, st, P);     \
+    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false, LMV>), grid, block, lds, st, P);
         if (P.H == 8) {
-            HX_R16(8)
+            if (lm) {
+                HX_R16(8, true)
+            } else {
+                HX_R16(8, false)
+            }
         } else if (P.H == 7) {
-            HX_R16(7)
+            if (lm) {
+                HX_R16(7, true)
+            } else {
+                HX_R16(7, false)
+            }
         } else {
-            HX_R16(6)
+            if (lm) {
+                HX_R16(6, true)
+            } else {
+                HX_R16(6, false)
+            }
         }
 #undef HX_R16
     } else {
