@@ -208,7 +208,8 @@ def main():
                    "ntt_limb_GBps": round(ntt_limb_gbs, 1), "intt_limb_GBps": round(intt_limb_gbs, 1),
                    "keyswitch_algorithmic_bytes": batched.algorithmic_bytes() // B,
                    "keyswitch_chain_bytes": batched.chain_bytes() // B,
-                   "keyswitch_fused_bytes": batched.fused_bytes() // B},
+                   "keyswitch_fused_bytes": batched.fused_bytes() // B,
+                   "fused_tuned_shape": batched.fused.config()},
         "roofline": {"bound": "hbm",
                      "kernel": f"k_ntt_pass_r16<8,false>: forward NTT of {limbs_fwd} limbs of N={N}, "
                                f"{n_launch} launches per NTT",

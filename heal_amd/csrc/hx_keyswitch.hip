@@ -57,19 +57,74 @@ struct InnerArgs {
     int64_t n_tg, n_hi;
 };
 
-// (sum + a) % M with C++ truncation, fast for the usual non-negative case
-__device__ __forceinline__ int64_t seq_rem(const ModInfo& mi, int64_t s) {
-    if (s >= 0 && (uint64_t)s < mi.M) return s;
-    if (s >= 0 && mi.M <= (uint64_t(1) << 62) && (uint64_t)s < 2 * mi.M) return s - (int64_t)mi.M;
-    return cpp_rem(mi, s);
+// The forward values of a MODE_L64 limb enter the key inner product lazy
+// ([0, 4q)) when the sum of D products stays below q * 2^64: D * 4q * q <= q * 2^64.
+__device__ __forceinline__ bool ks_lazy_dot(const LimbInfo& L, int D) {
+    return limb_mode(L) == MODE_L64 && (uint64_t)(4 * D) <= ~0ull / L.q;
+}
+
+// Key inner product of one lane's 16 values of layout B:
+//     x[e] = (sum_d F_d[w] * key_d[e]) mod q,   w = r16_wB(e, lane),
+// F_d the forward values of digit d in LDS (rows SP apart), key words in
+// [0, q).  One 128-bit accumulation and one reduction when the sum stays below
+// q * 2^64 (`single`), else a reduction per digit.  DC > 0: D = DC at compile
+// time, so that all key loads of a lane can be in flight together.
+template <int H, int DC>
+__device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, const int64_t* kp, int64_t dstride,
+                                       int lane, const ModInfo& mi, int D, bool single) {
+    constexpr int SP = (1 << H) + 1;
+    constexpr int TL = (1 << H) / 16;
+    if (DC > 0) D = DC;
+    if (DC > 0 && single) {
+        // the key words of CH registers are requested together, then used
+        constexpr int CH = DC <= 3 ? 16 : 8;
+#pragma unroll
+        for (int e0 = 0; e0 < 16; e0 += CH) {
+            uint64_t kv[DC > 0 ? DC : 1][CH];
+#pragma unroll
+            for (int d = 0; d < DC; ++d)
+#pragma unroll
+                for (int i = 0; i < CH; ++i) kv[d][i] = (uint64_t)kp[d * dstride + (e0 + i) * TL];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int w = r16_wB<H>(e0 + i, lane);
+                u128 acc = 0;
+#pragma unroll
+                for (int d = 0; d < DC; ++d) acc += (u128)xf[d * SP + w] * kv[d][i];
+                x[e0 + i] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
+            }
+        }
+    } else if (single) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int w = r16_wB<H>(e, lane);
+            u128 acc = 0;
+#pragma unroll
+            for (int d = 0; d < (DC > 0 ? DC : 1); ++d)
+                acc += (u128)xf[d * SP + w] * (uint64_t)kp[d * dstride + e * TL];
+            for (int d = DC > 0 ? DC : 1; d < D; ++d) acc += (u128)xf[d * SP + w] * (uint64_t)kp[d * dstride + e * TL];
+            x[e] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
+        }
+    } else {
+        const uint64_t q = mi.M;
+        for (int e = 0; e < 16; ++e) {
+            const int w = r16_wB<H>(e, lane);
+            uint64_t s = 0;
+            for (int d = 0; d < D; ++d) {
+                const uint64_t p = mulmod(mi, reduce64(mi, xf[d * SP + w]), (uint64_t)kp[d * dstride + e * TL]);
+                s = s >= q - p ? s - (q - p) : s + p;
+            }
+            x[e] = s;
+        }
+    }
 }
 
 // K2 on the radix-16 register layouts (see k_ntt_pass_r16).  A workgroup
 // takes the D digit columns of G limbs of one instance and one value `hi` of
 // the high index bits (rows hi*W .. hi*W + W - 1):
 //   forward  sub-problem s = d + D*gi: load in layout A, stages, LDS
-//            exchange, stages in layout B, full reduction, values to LDS
-//            in natural order;
+//            exchange, stages in layout B (reduced to [0, q) unless the inner
+//            product takes lazy values), values to LDS in natural order;
 //   inverse  sub-problem s = c + 2*gi: the key inner product for the values
 //            of layout B, stages, LDS exchange, stages in layout A, store.
 // The prepared key holds window index wB(e, lane) of row block hi at
@@ -120,7 +175,7 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
 #pragma unroll
             for (int e = 0; e < 16; ++e) x[e] = xs[r16_wB<H>(e, lane)];
             r16_stages_any<H, false, 1>(x, lane, twrow, hi, A.logm, A.hhi, L, mode);
-            if (mode == MODE_L64) {
+            if (mode == MODE_L64 && !ks_lazy_dot(L, D)) {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], L.q);
             }
@@ -131,13 +186,13 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
         __syncthreads();
     }
 
-    // ---- key inner product and sum over digits, in the chain's integer semantics:
+    // ---- key inner product and sum over digits.  The chain computes
     //      prod_d = (T_DP(D_d) * T_DP(key_d)) % q      (modmul_ttt)
     //      sum    = (sum + prod_d) % q, d = 0..D-1     (axis_modsum)
     //      INTT input = residue of sum                  (intt load)
-    // For key words in [0, q) this is (sum_d D_d * key_d) mod q, accumulated in
-    // 128 bits and reduced once when D * q < 2^64; other key words take the
-    // exact step-by-step path.
+    // and the residue of sum is (sum_d D_d * key_d) mod q whatever the signs and
+    // sizes of the key words, so the prepared key holds their residues and the
+    // sum is accumulated exactly in 128 bits.
     const bool act = tid < NI * TL;
     const int s = act ? tid % NI : 0, lane = act ? tid / NI : 0;
     const int c = s & 1, gi = s >> 1;
@@ -145,44 +200,21 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
     const bool valid = act && t < A.k;
     if (t >= A.k) t = A.k - 1;
     const LimbInfo& L = A.limbs[t];
-    const uint64_t q = L.q;
     const int mode = limb_mode(L);
     uint64_t x[16] = {};
     if (valid) {
         const ModInfo mi = limb_modinfo(L);
-        const bool one_reduction = (uint64_t)D <= ~0ull / q;
+        const bool single = ks_lazy_dot(L, D) || (uint64_t)D <= ~0ull / L.q;
         const int64_t kb = A.key_b == 1 ? 0 : b;
         const int64_t* kp = A.key + ((kb * 2 + c) * A.key_k + t) * A.m + row0 + lane;
         const int64_t dstride = A.key_b * 2 * A.key_k * A.m;
         const uint64_t* xf = lds + (int64_t)(D * gi) * SP;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int w = r16_wB<H>(e, lane);
-            u128 acc = 0;
-            bool canon = one_reduction;
-            for (int d = 0; d < D; ++d) {
-                const int64_t kv = kp[d * dstride + e * TL];
-                canon = canon && kv >= 0 && (uint64_t)kv < q;
-                acc += (u128)xf[d * SP + w] * (uint64_t)kv;
-            }
-            if (canon) {
-                x[e] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
-            } else {
-                int64_t sum = 0;
-                for (int d = 0; d < D; ++d) {
-                    const int64_t kv = kp[d * dstride + e * TL];
-                    const uint64_t xv = xf[d * SP + w];
-                    const int64_t prod = (kv >= 0 && (uint64_t)kv < q) ? (int64_t)mulmod(mi, xv, (uint64_t)kv)
-                                                                       : cpp_mulmod(mi, (int64_t)xv, kv);
-                    sum = seq_rem(mi, (int64_t)((uint64_t)sum + (uint64_t)prod));
-                }
-                uint64_t v = (uint64_t)sum;
-                if (sum < 0) {
-                    const uint64_t r = reduce64(mi, uabs64(sum));
-                    v = r ? q - r : 0;
-                }
-                x[e] = v;
-            }
+        switch (D) {
+            case 1: ks_dot<H, 1>(x, xf, kp, dstride, lane, mi, D, single); break;
+            case 2: ks_dot<H, 2>(x, xf, kp, dstride, lane, mi, D, single); break;
+            case 3: ks_dot<H, 3>(x, xf, kp, dstride, lane, mi, D, single); break;
+            case 4: ks_dot<H, 4>(x, xf, kp, dstride, lane, mi, D, single); break;
+            default: ks_dot<H, 0>(x, xf, kp, dstride, lane, mi, D, single); break;
         }
     }
 
@@ -219,13 +251,15 @@ __global__ __launch_bounds__(512) void k_ks_inner_multi(const InnerArgs* list, c
     ks_inner<H>(list[i], (int64_t)blockIdx.x - starts[i], starts[i + 1] - starts[i]);
 }
 
-// key[((d*B + b)*2 + c)*k + t][hi*W + e*TL + lane] = evk[d, b, c, rowmap[j], t]
-// with j = hi*W + r16_wB(e, lane), W = 2^H the inner window (H = 6..8)
+// key[((d*B + b)*2 + c)*k + t][hi*W + e*TL + lane] = evk[d, b, c, rowmap[j], t] mod q_t
+// (the residue in [0, q_t)) with j = hi*W + r16_wB(e, lane), W = 2^H the inner
+// window (H = 6..8)
 struct KeyArgs {
     const int64_t* evk;
     int64_t s[5];  // strides of the [D, B, 2, m, k] view
     int64_t* key;
     const int32_t* rowmap;  // null: identity
+    const LimbInfo* limbs;  // key words are stored as residues mod q_t
     int64_t D, B, m, k;
     int32_t H;
 };
@@ -246,7 +280,8 @@ __global__ void k_ks_prepare_key(KeyArgs K) {
         const int w = ((reg >> nb) << (2 * nb)) | (lane << nb) | (reg & ((1 << nb) - 1));
         const int64_t jw = (j & ~wmask) | w;
         int64_t u = K.rowmap ? (int64_t)K.rowmap[jw] : jw;
-        K.key[e] = K.evk[d * K.s[0] + b * K.s[1] + c * K.s[2] + u * K.s[3] + t * K.s[4]];
+        const int64_t v = K.evk[d * K.s[0] + b * K.s[1] + c * K.s[2] + u * K.s[3] + t * K.s[4]];
+        K.key[e] = (int64_t)residue(limb_modinfo(K.limbs[t]), v);
     }
 }
 
@@ -436,6 +471,7 @@ void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, vo
     for (int i = 0; i < 5; ++i) K.s[i] = evk->strides[i];
     K.key = (int64_t*)key;
     K.rowmap = plan->fwd->perm_kind == PERM_MAP ? plan->fwd->rowmap : nullptr;
+    K.limbs = plan->fwd->limbs;
     K.D = plan->D;
     K.B = evk->dims[1];
     K.m = plan->m;
@@ -638,18 +674,24 @@ static std::mutex g_tune_mu;
 static std::map<std::pair<int64_t, int>, int> g_tuned_hlo;
 static std::map<std::tuple<int64_t, int64_t, int>, int> g_tuned_group;
 
+// Fastest of five timed key-switches after one warm-up (the minimum is the
+// least noisy estimate of a kernel sequence's cost on a shared machine).
 static float time_keyswitch(hx_keyswitch_plan* plan, const hx_view* x, void* key, size_t key_bytes, const hx_view* out,
                             hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     KsArgs K;
     keyswitch_build(plan, x, key, key_bytes, -1, -1, nullptr, out, st, K);
     keyswitch_launch(K, st);  // warm-up
-    check_hip(hipEventRecord(e0, st), "hipEventRecord");
-    for (int r = 0; r < 3; ++r) keyswitch_launch(K, st);
-    check_hip(hipEventRecord(e1, st), "hipEventRecord");
-    check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
-    float ms = 0;
-    check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    return ms;
+    float best = 1e30f;
+    for (int r = 0; r < 5; ++r) {
+        check_hip(hipEventRecord(e0, st), "hipEventRecord");
+        keyswitch_launch(K, st);
+        check_hip(hipEventRecord(e1, st), "hipEventRecord");
+        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        best = std::min(best, ms);
+    }
+    return best;
 }
 
 static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
@@ -676,8 +718,10 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
         plan->G = G;
         return;
     }
-    // scratch key-switch: B instances of zeros (the timing does not depend on values)
-    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t(1) << 21) / (m * k)));
+    // scratch key-switch: B instances of zeros (the timing does not depend on
+    // values), enough instances that every candidate fills the GPU: at
+    // m * k = 2^16 * 24 that is 8 instances (about 0.9 GB of scratch)
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t(1) << 24) / (m * k)));
     const size_t xb = (size_t)B * m * k * 8, ob = 2 * xb, kb = (size_t)D * B * 2 * k * m * 8;
     void* mem = nullptr;
     check_hip(hipMallocAsync(&mem, xb + ob + kb, st), "hipMallocAsync(tuning)");
@@ -897,6 +941,14 @@ int hx_keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch, size_t*
     HX_API_BEGIN
     if (!bytes) raise(HX_ERR_INVALID_ARGUMENT, "bytes pointer must not be null.");
     *bytes = keyswitch_key_bytes(plan, batch);
+    HX_API_END
+}
+
+int hx_keyswitch_plan_config(const hx_keyswitch_plan* plan, int32_t* low_bits, int32_t* limb_group) {
+    HX_API_BEGIN
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    if (low_bits) *low_bits = plan->hlo;
+    if (limb_group) *limb_group = plan->G;
     HX_API_END
 }
 

@@ -193,6 +193,11 @@ typedef struct hx_keyswitch_plan hx_keyswitch_plan;
 int hx_keyswitch_plan_create(hx_keyswitch_plan** out, const hx_ntt_plan* fwd, const hx_ntt_plan* inv,
                              const hx_view* perms, uint64_t dnum, uint64_t base_bits, hx_stream stream);
 int hx_keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch, size_t* bytes);
+/* The tuned shape of a plan: low_bits = index bits transformed inside the
+ * middle kernel (the rest in the first and last), limb_group = limbs per
+ * middle-kernel workgroup.  Either pointer may be NULL.  No counterpart in the
+ * reference. */
+int hx_keyswitch_plan_config(const hx_keyswitch_plan* plan, int32_t* low_bits, int32_t* limb_group);
 int hx_keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hx_stream stream);
 int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                       const hx_view* out, hx_stream stream);
