@@ -36,6 +36,7 @@ struct hx_keyswitch_plan {
     int G;            // limbs per K2 workgroup (tuned)
     void* mem;
     int32_t* galmap;  // [k][m]: output row of coefficient j of limb t
+    int gal_shared;   // every limb has the same map (K3 then reads row 0 only)
     mutable void* ws;
     mutable size_t ws_bytes;
 };
@@ -310,6 +311,12 @@ __global__ void k_ks_gal_map(PlanSrc perms, int64_t k, int64_t m, const int32_t*
     }
 }
 
+// counts the entries of galmap [k][m] that differ from limb 0's row
+__global__ void k_ks_gal_same(const int32_t* galmap, int64_t k, int64_t m, unsigned long long* flags) {
+    for (int64_t e = m + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k * m; e += (int64_t)gridDim.x * blockDim.x)
+        if (galmap[e] != galmap[e % m]) atomicAdd(&flags[3], 1ull);
+}
+
 // forward and inverse plans must agree on moduli and permutation
 __global__ void k_ks_compare(const LimbInfo* a, const LimbInfo* b, int64_t k, const int32_t* ma, const int32_t* mb,
                              int64_t m, unsigned long long* flags) {
@@ -376,13 +383,15 @@ hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_pl
         hipLaunchKernelGGL(k_ks_gal_map, dim3(g), dim3(256), 0, st, ps, k, m, counts, plan->galmap, flags);
         hipLaunchKernelGGL(k_ks_compare, dim3(grid_for(k + m, 256, 1024)), dim3(256), 0, st, fwd->limbs, inv->limbs, k,
                            fmap ? fwd->rowmap : nullptr, imap ? inv->rowmap : nullptr, m, flags);
+        hipLaunchKernelGGL(k_ks_gal_same, dim3(g), dim3(256), 0, st, plan->galmap, k, m, flags);
         check_hip(hipGetLastError(), "keyswitch plan launch");
-        unsigned long long h[3];
+        unsigned long long h[4];
         check_hip(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
         check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
         if (h[0]) raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
         if (h[1]) raise(HX_ERR_INVALID_ARGUMENT, "the fused key-switch needs every row of perms to be a permutation");
         if (h[2]) raise(HX_ERR_INVALID_ARGUMENT, "forward and inverse plans differ in moduli or permutation");
+        plan->gal_shared = h[3] == 0;
         keyswitch_tune(plan, st);
     } catch (...) {
         (void)hipStreamSynchronize(st);
@@ -635,6 +644,7 @@ bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void
     P3.bot_count = int64_t(1) << plan->hlo;
     P3.scale_out = 1;
     if (row_map) P3.dst_map = row_map;  // one Galois permutation for every limb
+    else if (plan->gal_shared) P3.dst_map = plan->galmap;  // limb 0's row serves every limb
     else P3.dst_map_limb = plan->galmap;
     P3.limbs = I->limbs;
     P3.tw = I->tw;

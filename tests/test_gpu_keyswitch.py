@@ -90,6 +90,28 @@ def test_fused_keyswitch_negative_key_matches_chain(gpu):
     ks.close()
 
 
+def test_fused_keyswitch_per_limb_permutations(gpu):
+    """A different permutation for every limb (the per-limb row scatter in the last
+    kernel) and the same one for every limb (one shared row map) both equal the
+    six-call chain, whose permute applies perms [L, N] row by row."""
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    for shared in (False, True):
+        print(f"[fused perms] shared={shared}", flush=True)
+        ks = KeySwitchBatch(n=4096, limbs=5, dnum=2, base_bits=30, batch=2, seed=91)
+        g = torch.Generator().manual_seed(5)
+        rows = [torch.randperm(ks.n, generator=g) for _ in range(1 if shared else ks.L)]
+        ks.perms = torch.stack([rows[0 if shared else t] for t in range(ks.L)]).to(ks.perms)
+        ks.run()
+        torch.cuda.synchronize()
+        want = ks.out.clone()
+        out = torch.zeros_like(want)
+        ks.run_fused(out)
+        torch.cuda.synchronize()
+        assert torch.equal(out, want), shared
+        ks.close()
+
+
 def test_fused_keyswitch_full_size_matches_plain_chain(gpu):
     from heal_amd.keyswitch import KeySwitchBatch
 
