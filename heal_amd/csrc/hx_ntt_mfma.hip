@@ -48,7 +48,8 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int kRows = 256;
 constexpr int kBlockCols = 32;            // columns of the right operand per workgroup
 constexpr int kColBlocks = kRows / kBlockCols;
-constexpr int kThreads4 = 512;            // 8 waves: 32 rows of the product each
+constexpr int kRowHalves = 2;             // a workgroup computes 128 of the 256 product rows
+constexpr int kThreads4 = 512;            // 8 waves: 16 rows x 32 columns of the product each
 
 struct Ntt4Args {
     const int64_t* a;      // input element (p, i, t) at a + p*a_sp + t*a_st + i*a_si
@@ -80,14 +81,16 @@ __device__ __forceinline__ uint64_t reduce_in(int64_t x, uint64_t q) {
 template <int STEP>
 __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
     __shared__ uint4 bs[2 * 4 * 4 * 64];  // right operand planes: [c-tile][plane][k-chunk][lane]
-    // workgroups of one limb are adjacent (they share the left matrix in L2)
+    // workgroups of one limb are adjacent (they share the left matrix in L2), and
+    // so are the two row halves of one column block (they share its right operand)
     // (blocks are dispatched to the 8 XCDs round-robin: give each XCD a contiguous range)
-    const int64_t nblk = A.P * A.K * kColBlocks;
+    const int64_t nblk = A.P * A.K * kColBlocks * kRowHalves;
     const int64_t xcd = (int64_t)blockIdx.x % 8, rem = nblk % 8;
     const int64_t blk = xcd * (nblk / 8) + (xcd < rem ? xcd : rem) + (int64_t)blockIdx.x / 8;
-    const int64_t t = blk / (A.P * kColBlocks);
-    const int64_t p = (blk / kColBlocks) % A.P;
-    const int cb = (int)(blk % kColBlocks);
+    const int64_t t = blk / (A.P * kColBlocks * kRowHalves);
+    const int64_t p = (blk / (kColBlocks * kRowHalves)) % A.P;
+    const int cb = (int)((blk / kRowHalves) % kColBlocks);
+    const int rh = (int)(blk % kRowHalves);
     const int64_t col = p * A.K + t;
     const uint64_t* rd = A.red + t * 10;
     const uint64_t q = rd[0], mu = rd[1];
@@ -126,19 +129,33 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
     }
     __syncthreads();
 
-    // ---- 32 rows x 32 columns per wave: 2 x 2 tiles, 7 accumulators each
+    // ---- 16 rows x 32 columns per wave: 1 x 2 tiles, 7 accumulators each
     const int wave = tid / 64, lane = tid % 64;
-    v4i acc[2][2][7];
+    const int ut = rh * 8 + wave;  // the wave's 16-row tile of the product
+    v4i acc[2][7];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int s = 0; s < 7; ++s) acc[a][b][s] = v4i{0, 0, 0, 0};
+        for (int s = 0; s < 7; ++s) acc[b][s] = v4i{0, 0, 0, 0};
     const uint4* wl = A.w + t * (4 * 16 * 4 * 64);
+    v4i am[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const uint4 x = wl[((l * 16 + ut) * 4 + 0) * 64 + lane];
+        am[l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+    }
 #pragma unroll
     for (int kq = 0; kq < 4; ++kq) {
         __builtin_amdgcn_sched_barrier(0);  // one k chunk of operands live at a time
+        // the next chunk's left operand is requested before this chunk's products
+        v4i an[4];
+        if (kq + 1 < 4) {
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const uint4 x = wl[((l * 16 + ut) * 4 + kq + 1) * 64 + lane];
+                an[l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+            }
+        }
         v4i bm[2][4];
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct)
@@ -148,21 +165,15 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
                 bm[ct][l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
             }
 #pragma unroll
-        for (int ut = 0; ut < 2; ++ut) {
-            v4i am[4];
+        for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                const uint4 x = wl[((l * 16 + (2 * wave + ut)) * 4 + kq) * 64 + lane];
-                am[l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
-            }
+            for (int la = 0; la < 4; ++la)
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
+                for (int lb = 0; lb < 4; ++lb)
+                    acc[ct][la + lb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[la], bm[ct][lb], acc[ct][la + lb], 0, 0, 0);
+        if (kq + 1 < 4) {
 #pragma unroll
-                for (int la = 0; la < 4; ++la)
-#pragma unroll
-                    for (int lb = 0; lb < 4; ++lb)
-                        acc[ut][ct][la + lb] =
-                            __builtin_amdgcn_mfma_i32_16x16x64_i8(am[la], bm[ct][lb], acc[ut][ct][la + lb], 0, 0, 0);
+            for (int l = 0; l < 4; ++l) am[l] = an[l];
         }
     }
 
@@ -171,18 +182,17 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
     uint64_t R[7];
 #pragma unroll
     for (int s = 0; s < 7; ++s) R[s] = rd[2 + s];
-#pragma unroll
-    for (int ut = 0; ut < 2; ++ut)
+    {
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
-            const int row0 = (2 * wave + ut) * 16 + 4 * (lane / 16);
+            const int row0 = ut * 16 + 4 * (lane / 16);
             const int gc = cb * kBlockCols + ct * 16 + lane % 16;
             uint32_t r4[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 uint64_t v = 0;
 #pragma unroll
-                for (int s = 0; s < 7; ++s) v += (uint64_t)(uint32_t)acc[ut][ct][s][j] * R[s];
+                for (int s = 0; s < 7; ++s) v += (uint64_t)(uint32_t)acc[ct][s][j] * R[s];
                 v = barrett(v, q, mu);
                 if (STEP == 1) {
                     const uint64_t tw = A.tmat[(t * kRows + row0 + j) * kRows + gc];  // T[u2][i1]
@@ -199,6 +209,7 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
                 for (int j = 0; j < 4; ++j) ob[((int64_t)(row0 + j) * kRows + gc) * A.o_si] = (int64_t)r4[j];
             }
         }
+    }
 }
 
 }  // namespace
@@ -276,7 +287,7 @@ void ntt_mfma_exec(const hx_ntt_mfma_plan* plan, const hx_view* a, const hx_view
     A.red = plan->red;
     A.P = P;
     A.K = K;
-    const int64_t blocks = P * K * kColBlocks;
+    const int64_t blocks = P * K * kColBlocks * kRowHalves;
     if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "too many columns for one launch");
     if (views_overlap(a, out) && !(a->data == out->data)) raise(HX_ERR_INVALID_ARGUMENT, "a and result overlap");
     A.w = plan->w2;
