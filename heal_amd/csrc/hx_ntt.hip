@@ -98,9 +98,11 @@ hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm,
             unsigned g = grid_for(m, 256, 4096);
             hipLaunchKernelGGL(k_plan_perm_count, dim3(g), dim3(256), 0, st, plan_src(perm), m, counts, flags);
             hipLaunchKernelGGL(k_plan_perm_check, dim3(g), dim3(256), 0, st, counts, m, flags);
+            if (plan->logm <= 30)
+                hipLaunchKernelGGL(k_plan_bitrev, dim3(g), dim3(256), 0, st, plan_src(perm), m, plan->logm, flags);
         }
         check_hip(hipGetLastError(), "plan launch");
-        unsigned long long hflags[4];
+        unsigned long long hflags[5];
         check_hip(hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(flags)");
         check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
         if (hflags[1]) raise(HX_ERR_OUT_OF_RANGE, "Index out of bounds.");
@@ -109,6 +111,7 @@ hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm,
             plan->perm_kind = PERM_IDENTITY;
         } else if (hflags[3] == 0) {
             plan->perm_kind = PERM_MAP;
+            plan->perm_bitrev = m > 1 && plan->logm <= 30 && hflags[4] == 0;
             hipLaunchKernelGGL(k_plan_map, dim3(grid_for(m, 256, 4096)), dim3(256), 0, st, plan_src(perm), m,
                                plan->rowmap, 0);
         } else if (!inverse) {
@@ -280,6 +283,13 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         // a tile of whole columns whose rows are contiguous: lanes of a sub-problem
         // take adjacent rows (coalesced), not adjacent columns
         P.lane_major = col_tiles && rows_contiguous;
+        // the bit-reversal row map of rows-contiguous columns, fused into the
+        // pass over the low bits (the forward's last, the inverse's first): tiles
+        // of tops whose permuted rows are adjacent
+        const bool map_side = plan->inverse ? (P.src_map != nullptr && P.src.row_stride == 1)
+                                            : (P.dst_map != nullptr && P.dst.row_stride == 1);
+        P.top_bitrev = plan->perm_bitrev && rows_contiguous && map_side && P.b0 == 0 && r16_window(H) &&
+                       logm - H >= 5 && npass > 1;
         launch_pass(P, st);
         done_bits += H;
     }

@@ -31,6 +31,7 @@ struct hx_ntt_plan {
     int64_t m;
     int64_t k;
     int perm_kind;        // PERM_* below
+    int perm_bitrev;      // perm is the bit reversal of log m bits (a PERM_MAP that is its own inverse)
     void* mem;            // single device allocation
     // Intermediate buffer between passes.  A plan executes on one stream at a
     // time (like an FFT plan); the workspace grows on first use.
@@ -148,6 +149,14 @@ __global__ void k_plan_perm_check(const int32_t* counts, int64_t m, unsigned lon
         if (counts[j] != 1) atomicAdd(&flags[3], 1ull);
 }
 
+// counts the entries of perm that differ from the bit reversal of log m bits
+__global__ void k_plan_bitrev(PlanSrc perm, int64_t m, int logm, unsigned long long* flags) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < m; u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t br = logm ? (int64_t)(__brev((uint32_t)u) >> (32 - logm)) : 0;
+        if (ld_elem(perm.ptr, u * perm.s0, perm.elem32) != br) atomicAdd(&flags[4], 1ull);
+    }
+}
+
 // mode 0: map[perm[u]] = u; mode 1: map[u] = perm[u]; mode 2: map[perm[u]] = max u (map pre-filled with -1)
 __global__ void k_plan_map(PlanSrc perm, int64_t m, int32_t* map, int mode) {
     for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < m; u += (int64_t)gridDim.x * blockDim.x) {
@@ -195,6 +204,8 @@ struct PassArgs {
     int32_t scale_out;
     int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 4q) forward, [0, 2q) inverse)
     int32_t lane_major;      // radix-16 passes: a sub-problem's lanes are adjacent threads (rows contiguous in memory)
+    int32_t top_bitrev;      // radix-16 pass over the low bits (b0 = 0) of rows-contiguous columns whose row map is
+                             // the bit reversal: tiles of 32 "top" values with adjacent bit reversals (r16_pass_brt)
     // digit source (src.kind == 3): column axis `gd_axis` is the digit index d,
     // the value loaded is digit d of x in base 2^gd_w; elements of digit 0 whose
     // value is not exhausted after gd_D digits are counted in gd_status
@@ -769,6 +780,91 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
     r16_pass<H, INV, LM>(P, blockIdx.x, gridDim.x);
 }
 
+// The radix-16 pass over the low index bits (b0 = 0) of one rows-contiguous
+// column whose row map is the bit reversal of log m bits: the forward pass
+// stores x[j] at row brev(j), the inverse pass loads x[j] from row brev(j).  With
+// j = top * 2^H + w, brev(j) = brev_H(w) * 2^(log m - H) + brev(top), so a tile
+// of the 32 tops whose bit reversals are v0 .. v0 + 31 touches 32 adjacent
+// permuted rows for every w.  The side of the pass at the permuted rows (the
+// forward store, the inverse load) runs with the 32 sub-problems on adjacent
+// threads; the other side with the 2^(H-4) lanes of one sub-problem on adjacent
+// threads (adjacent rows); the LDS exchange between the two rounds switches
+// the thread order.  The column, and so the limb, is the same for the tile.
+template <int H, bool INV>
+__device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int64_t nblk) {
+    constexpr int W = 1 << H;
+    constexpr int TL = W / 16;
+    constexpr int S = 32;
+    constexpr int SP = W + 1;
+    constexpr int L_IN = INV ? 1 : 0;
+    constexpr int L_OUT = 1 - L_IN;
+    extern __shared__ uint64_t xs[];  // [S][SP]
+    const int ntb = P.logm - H;       // bits of top
+    const int64_t tile = xcd_tile(blk, nblk);
+    const int64_t col = tile % P.ncols;
+    const int64_t v0 = (tile / P.ncols) * S;  // bit reversals of the tile's tops: v0 .. v0 + 31
+    int64_t c = col, coord[3];
+    coord[0] = c % P.cdim[0];
+    c /= P.cdim[0];
+    coord[1] = c % P.cdim[1];
+    coord[2] = c / P.cdim[1];
+    const int64_t lcoord[3] = {col, 0, 0};
+    const int64_t src_off = P.src.kind == 2 ? col_offset(P.src, lcoord) : col_offset(P.src, coord);
+    const int64_t dst_off = P.dst.kind == 2 ? col_offset(P.dst, lcoord) : col_offset(P.dst, coord);
+    const int limb = (int)coord[P.limb_axis];
+    const LimbInfo& L = P.limbs[limb];
+    const uint64_t q = L.q;
+    const int mode = limb_mode(L);
+    const ulonglong2* twrow = (const ulonglong2*)(P.tw + 2 * (int64_t)limb * P.m);
+    const int tid = threadIdx.x;
+    // thread orders: "rows" (lanes adjacent) and "perm" (sub-problems adjacent)
+    const int s_r = tid / TL, lane_r = tid % TL;
+    const int s_p = tid % S, lane_p = tid / S;
+    const int s1 = INV ? s_p : s_r, lane1 = INV ? lane_p : lane_r;
+    const int s2 = INV ? s_r : s_p, lane2 = INV ? lane_r : lane_p;
+    const int64_t top1 = (int64_t)(__brev((uint32_t)(v0 + s1)) >> (32 - ntb));
+    const int64_t top2 = (int64_t)(__brev((uint32_t)(v0 + s2)) >> (32 - ntb));
+    const int es_in = P.src.kind == 1 ? 4 : 8, es_out = P.dst.kind == 1 ? 4 : 8;
+    uint64_t x[16];
+
+    // ---- load: forward rows top * W + w; inverse rows brev(top * W + w)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int w = r16_w<H>(L_IN, e, lane1);
+        const int64_t row = INV ? ((int64_t)(__brev((uint32_t)w) >> (32 - H)) << ntb) + v0 + s1 : (top1 << H) + w;
+        const int64_t off = src_off + row * P.src.row_stride;
+        const int64_t xv = es_in == 4 ? (int64_t)((const int32_t*)P.src.ptr)[off] : ((const int64_t*)P.src.ptr)[off];
+        uint64_t v = (uint64_t)xv;
+        if (P.reduce_in && (xv < 0 || v >= q)) v = residue(limb_modinfo(L), xv);
+        x[e] = v;
+    }
+    r16_stages_any<H, INV, L_IN>(x, lane1, twrow, top1, P.logm, P.stage0, L, mode);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xs[s1 * SP + r16_w<H>(L_IN, e, lane1)] = x[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = xs[s2 * SP + r16_w<H>(L_OUT, e, lane2)];
+    r16_stages_any<H, INV, L_OUT>(x, lane2, twrow, top2, P.logm, P.stage0, L, mode, INV && P.scale_out);
+    if (!P.scale_out && !P.lazy_out && mode == MODE_L64) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], q);
+    }
+    // ---- store: forward rows brev(top * W + w); inverse rows top * W + w
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int w = r16_w<H>(L_OUT, e, lane2);
+        const int64_t row = INV ? (top2 << H) + w : ((int64_t)(__brev((uint32_t)w) >> (32 - H)) << ntb) + v0 + s2;
+        const int64_t off = dst_off + row * P.dst.row_stride;
+        if (es_out == 4) ((int32_t*)P.dst.ptr)[off] = (int32_t)x[e];
+        else ((uint64_t*)P.dst.ptr)[off] = x[e];
+    }
+}
+
+template <int H, bool INV>
+__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16_brt(PassArgs P) {
+    r16_pass_brt<H, INV>(P, blockIdx.x, gridDim.x);
+}
+
 // Index of the instance whose tile range [starts[i], starts[i + 1]) holds blk.
 __device__ __forceinline__ int instance_of(const int64_t* starts, int count, int64_t blk) {
     int lo = 0, hi = count - 1;
@@ -888,6 +984,11 @@ constexpr int kR16Tile = 32;  // sub-problems per workgroup of the radix-16 pass
 // Fills P.n_bot_tiles / P.n_col_tiles and returns the pass's workgroup count.
 int64_t pass_blocks(PassArgs& P) {
     const int64_t ntop = int64_t(1) << (P.logm - P.b0 - P.H);
+    if (P.top_bitrev) {  // one column and 32 tops per tile
+        const int64_t blocks = P.ncols * (ntop / kR16Tile);
+        if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
+        return blocks;
+    }
     P.n_bot_tiles = (P.bot_count + P.S_bot - 1) / P.S_bot;
     P.n_col_tiles = (P.ncols + P.S_cols - 1) / P.S_cols;
     const int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
@@ -922,6 +1023,24 @@ void launch_pass(PassArgs& P, hipStream_t st) {
     const int W = 1 << P.H;
     const int S = P.S_cols * P.S_bot;
     const int64_t blocks = pass_blocks(P);
+    if (P.top_bitrev) {
+        if (!r16_window(P.H) || P.b0 != 0 || P.logm - P.H < 5) raise(HX_ERR_RUNTIME, "internal: bit-reversal tiles");
+        const size_t lds = (size_t)kR16Tile * (W + 1) * 8;
+        const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
+#define HX_R16B(HH)                                                                                  \
+    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16_brt<HH, true>), grid, block, lds, st, P);      \
+    else hipLaunchKernelGGL((k_ntt_pass_r16_brt<HH, false>), grid, block, lds, st, P);
+        if (P.H == 8) {
+            HX_R16B(8)
+        } else if (P.H == 7) {
+            HX_R16B(7)
+        } else {
+            HX_R16B(6)
+        }
+#undef HX_R16B
+        check_hip(hipGetLastError(), "ntt pass launch");
+        return;
+    }
     if (r16_window(P.H)) {
         if (S != kR16Tile) raise(HX_ERR_RUNTIME, "internal: radix-16 tile size");
         const bool lm = P.lane_major != 0;

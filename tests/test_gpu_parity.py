@@ -233,6 +233,55 @@ def test_ntt_intt_parity(gpu, m, bits, dtype, layout, perm_kind, table):
         assert torch.equal(ig.cpu(), a)  # round trip
 
 
+LIMB_MAJOR_GRID = [
+    # m, bits, dtype, perm (input and result both limb-major)
+    (4096, [30, 29, 28], torch.int32, "bitrev"),
+    (4096, [60, 30], torch.int64, "bitrev"),
+    (32768, [59, 27], torch.int64, "bitrev"),
+    (65536, [60, 28, 50], torch.int64, "bitrev"),
+    (65536, [60, 45], torch.int64, "random"),
+]
+
+
+@pytest.mark.parametrize("m,bits,dtype,perm_kind", LIMB_MAJOR_GRID,
+                         ids=[f"m{c[0]}-{c[3]}-{str(c[2])[6:]}" for c in LIMB_MAJOR_GRID])
+def test_ntt_intt_limb_major_results(gpu, m, bits, dtype, perm_kind):
+    """Limb-major input and result views: the pass at the permuted rows takes
+    tiles of bit-reversed rows (bit-reversal perm) or scatters (other perms).
+    The int64 inverse also runs on unreduced entries (x + q), which it reduces
+    on load (the reference's signed remainders differ there, so the check is
+    that they transform like their residues)."""
+    gen = torch.Generator().manual_seed(m + 7)
+    primes = P.ntt_primes(m, bits)
+    prm, perm, tw, itw, a = _ntt_case(m, primes, dtype, "limbmajor", perm_kind, "psi", gen, l=3, r=1)
+    print(f"[ntt limb-major] m={m} bits={bits} {perm_kind} {dtype}", flush=True)
+    O, G = OracleBackend(), GpuBackend()
+
+    def limb_major_cuda(t):  # [l, m, r, k] view over [k, l, r, m] storage
+        return t.permute(3, 0, 2, 1).contiguous().cuda().permute(1, 3, 2, 0)
+
+    ro = torch.zeros_like(a)
+    assert O.ntt(a, prm.p, perm, tw, ro) is None
+    rg = limb_major_cuda(torch.zeros_like(a))
+    assert G.ntt(limb_major_cuda(a), prm.p.cuda(), perm.cuda(), tw.cuda(), rg) is None
+    assert torch.equal(rg.cpu(), ro)
+    # inverse from limb-major input, against the oracle and as a round trip
+    io = torch.zeros_like(a)
+    assert O.intt(ro, prm.p, perm, itw, prm.m_inv, io) is None
+    ig = limb_major_cuda(torch.zeros_like(a))
+    assert G.intt(limb_major_cuda(ro), prm.p.cuda(), perm.cuda(), itw.cuda(), prm.m_inv.cuda(), ig) is None
+    assert torch.equal(ig.cpu(), io)
+    # entries in [q, 2q) (int64; the reference adds int32 operands in int32) are
+    # reduced on load: the same residues give the same transform
+    if dtype == torch.int64:
+        x = ro.clone()
+        x[:, ::7] += prm.p.view(1, 1, 1, -1).expand_as(x[:, ::7])
+        iu = limb_major_cuda(torch.zeros_like(a))
+        assert G.intt(limb_major_cuda(x), prm.p.cuda(), perm.cuda(), itw.cuda(), prm.m_inv.cuda(), iu) is None
+        assert torch.equal(iu.cpu(), io)
+    assert torch.equal(ig.cpu(), a)  # round trip
+
+
 def test_ntt_plan_reuse_and_inplace(gpu):
     from heal_amd import capi
 
