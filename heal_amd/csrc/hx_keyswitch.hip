@@ -58,6 +58,17 @@ struct InnerArgs {
     int64_t n_tg, n_hi;
 };
 
+// LDS rows of K2: one pad slot every 16 (rows of W + W/16), so that 16
+// adjacent lanes of one sub-problem and adjacent sub-problems of a wave both
+// spread over the banks in either register layout.
+template <int H>
+__device__ constexpr int ks_lds_row() {
+    return (1 << H) + (1 << H) / 16;
+}
+__device__ __forceinline__ int ks_slot(int w) {
+    return w + (w >> 4);
+}
+
 // The forward values of a MODE_L64 limb enter the key inner product lazy
 // ([0, 4q)) when the sum of D products stays below q * 2^64: D * 4q * q <= q * 2^64.
 __device__ __forceinline__ bool ks_lazy_dot(const LimbInfo& L, int D) {
@@ -73,7 +84,7 @@ __device__ __forceinline__ bool ks_lazy_dot(const LimbInfo& L, int D) {
 template <int H, int DC>
 __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, const int64_t* kp, int64_t dstride,
                                        int lane, const ModInfo& mi, int D, bool single) {
-    constexpr int SP = (1 << H) + 1;
+    constexpr int SP = ks_lds_row<H>();
     constexpr int TL = (1 << H) / 16;
     if (DC > 0) D = DC;
     if (DC > 0 && single) {
@@ -91,7 +102,7 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
                 const int w = r16_wB<H>(e0 + i, lane);
                 u128 acc = 0;
 #pragma unroll
-                for (int d = 0; d < DC; ++d) acc += (u128)xf[d * SP + w] * kv[d][i];
+                for (int d = 0; d < DC; ++d) acc += (u128)xf[d * SP + ks_slot(w)] * kv[d][i];
                 x[e0 + i] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
             }
         }
@@ -102,8 +113,8 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
             u128 acc = 0;
 #pragma unroll
             for (int d = 0; d < (DC > 0 ? DC : 1); ++d)
-                acc += (u128)xf[d * SP + w] * (uint64_t)kp[d * dstride + e * TL];
-            for (int d = DC > 0 ? DC : 1; d < D; ++d) acc += (u128)xf[d * SP + w] * (uint64_t)kp[d * dstride + e * TL];
+                acc += (u128)xf[d * SP + ks_slot(w)] * (uint64_t)kp[d * dstride + e * TL];
+            for (int d = DC > 0 ? DC : 1; d < D; ++d) acc += (u128)xf[d * SP + ks_slot(w)] * (uint64_t)kp[d * dstride + e * TL];
             x[e] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
         }
     } else {
@@ -112,7 +123,7 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
             const int w = r16_wB<H>(e, lane);
             uint64_t s = 0;
             for (int d = 0; d < D; ++d) {
-                const uint64_t p = mulmod(mi, reduce64(mi, xf[d * SP + w]), (uint64_t)kp[d * dstride + e * TL]);
+                const uint64_t p = mulmod(mi, reduce64(mi, xf[d * SP + ks_slot(w)]), (uint64_t)kp[d * dstride + e * TL]);
                 s = s >= q - p ? s - (q - p) : s + p;
             }
             x[e] = s;
@@ -123,18 +134,22 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
 // K2 on the radix-16 register layouts (see k_ntt_pass_r16).  A workgroup
 // takes the D digit columns of G limbs of one instance and one value `hi` of
 // the high index bits (rows hi*W .. hi*W + W - 1):
-//   forward  sub-problem s = d + D*gi: load in layout A, stages, LDS
-//            exchange, stages in layout B (reduced to [0, q) unless the inner
-//            product takes lazy values), values to LDS in natural order;
+//   forward  sub-problem s = d + D*gi on thread s + NF*lane (sub-problems
+//            adjacent: a load covers adjacent columns of a row): load in
+//            layout A, stages, LDS exchange, stages in layout B (reduced to
+//            [0, q) unless the inner product takes lazy values), to LDS;
 //   inverse  sub-problem s = c + 2*gi: the key inner product for the values
-//            of layout B, stages, LDS exchange, stages in layout A, store.
+//            of layout B and the stages of layout B on thread lane + TL*s
+//            (lanes adjacent: a key load covers TL adjacent words, whole
+//            cache lines), LDS exchange, the stages of layout A and the
+//            store on thread s + NI*lane (adjacent columns again).
 // The prepared key holds window index wB(e, lane) of row block hi at
-// hi*W + e*TL + lane, so a key load of one register covers TL adjacent words.
+// hi*W + e*TL + lane.
 template <int H>
 __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_t nblk) {
     constexpr int W = 1 << H;
     constexpr int TL = W / 16;
-    constexpr int SP = W + 1;
+    constexpr int SP = ks_lds_row<H>();
     extern __shared__ uint64_t lds[];  // [max(NF, NI)][SP]
     const int D = A.D, G = A.G;
     const int NF = G * D, NI = 2 * G;
@@ -169,12 +184,12 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
         uint64_t* xs = lds + s * SP;
         if (act) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) xs[r16_wA<H>(e, lane)] = x[e];
+            for (int e = 0; e < 16; ++e) xs[ks_slot(r16_wA<H>(e, lane))] = x[e];
         }
         __syncthreads();
         if (valid) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) x[e] = xs[r16_wB<H>(e, lane)];
+            for (int e = 0; e < 16; ++e) x[e] = xs[ks_slot(r16_wB<H>(e, lane))];
             r16_stages_any<H, false, 1>(x, lane, twrow, hi, A.logm, A.hhi, L, mode);
             if (mode == MODE_L64 && !ks_lazy_dot(L, D)) {
 #pragma unroll
@@ -182,7 +197,7 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
             }
             // each thread rewrites exactly the slots it has just read
 #pragma unroll
-            for (int e = 0; e < 16; ++e) xs[r16_wB<H>(e, lane)] = x[e];
+            for (int e = 0; e < 16; ++e) xs[ks_slot(r16_wB<H>(e, lane))] = x[e];
         }
         __syncthreads();
     }
@@ -195,43 +210,53 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
     // sizes of the key words, so the prepared key holds their residues and the
     // sum is accumulated exactly in 128 bits.
     const bool act = tid < NI * TL;
-    const int s = act ? tid % NI : 0, lane = act ? tid / NI : 0;
-    const int c = s & 1, gi = s >> 1;
-    int64_t t = t0 + gi;
-    const bool valid = act && t < A.k;
-    if (t >= A.k) t = A.k - 1;
-    const LimbInfo& L = A.limbs[t];
-    const int mode = limb_mode(L);
     uint64_t x[16] = {};
-    if (valid) {
-        const ModInfo mi = limb_modinfo(L);
-        const bool single = ks_lazy_dot(L, D) || (uint64_t)D <= ~0ull / L.q;
-        const int64_t kb = A.key_b == 1 ? 0 : b;
-        const int64_t* kp = A.key + ((kb * 2 + c) * A.key_k + t) * A.m + row0 + lane;
-        const int64_t dstride = A.key_b * 2 * A.key_k * A.m;
-        const uint64_t* xf = lds + (int64_t)(D * gi) * SP;
-        switch (D) {
-            case 1: ks_dot<H, 1>(x, xf, kp, dstride, lane, mi, D, single); break;
-            case 2: ks_dot<H, 2>(x, xf, kp, dstride, lane, mi, D, single); break;
-            case 3: ks_dot<H, 3>(x, xf, kp, dstride, lane, mi, D, single); break;
-            case 4: ks_dot<H, 4>(x, xf, kp, dstride, lane, mi, D, single); break;
-            default: ks_dot<H, 0>(x, xf, kp, dstride, lane, mi, D, single); break;
+    {
+        const int s = act ? tid / TL : 0, lane = act ? tid % TL : 0;
+        const int c = s & 1, gi = s >> 1;
+        int64_t t = t0 + gi;
+        const bool valid = act && t < A.k;
+        if (t >= A.k) t = A.k - 1;
+        const LimbInfo& L = A.limbs[t];
+        const int mode = limb_mode(L);
+        if (valid) {
+            const ModInfo mi = limb_modinfo(L);
+            const bool single = ks_lazy_dot(L, D) || (uint64_t)D <= ~0ull / L.q;
+            const int64_t kb = A.key_b == 1 ? 0 : b;
+            const int64_t* kp = A.key + ((kb * 2 + c) * A.key_k + t) * A.m + row0 + lane;
+            const int64_t dstride = A.key_b * 2 * A.key_k * A.m;
+            const uint64_t* xf = lds + (int64_t)(D * gi) * SP;
+            switch (D) {
+                case 1: ks_dot<H, 1>(x, xf, kp, dstride, lane, mi, D, single); break;
+                case 2: ks_dot<H, 2>(x, xf, kp, dstride, lane, mi, D, single); break;
+                case 3: ks_dot<H, 3>(x, xf, kp, dstride, lane, mi, D, single); break;
+                case 4: ks_dot<H, 4>(x, xf, kp, dstride, lane, mi, D, single); break;
+                default: ks_dot<H, 0>(x, xf, kp, dstride, lane, mi, D, single); break;
+            }
+            // ---- inverse stages over the low bits (global stages 0 .. hlo-1), layout B
+            const ulonglong2* twrow = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
+            r16_stages_any<H, true, 1>(x, lane, twrow, hi, A.logm, 0, L, mode);
         }
+        __syncthreads();  // every read of the forward values is done
+        if (act) {
+            uint64_t* xs = lds + s * SP;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xs[ks_slot(r16_wB<H>(e, lane))] = x[e];
+        }
+        __syncthreads();
     }
 
-    // ---- inverse stages over the low bits (global stages 0 .. hlo-1)
+    // ---- stages of layout A and the store, sub-problems on adjacent threads
+    const int s = act ? tid % NI : 0, lane = act ? tid / NI : 0;
+    const int c = s & 1, gi = s >> 1;
+    const int64_t t = t0 + gi;
+    if (!act || t >= A.k) return;
+    const LimbInfo& L = A.limbs[t];
+    const int mode = limb_mode(L);
+    const uint64_t* xs = lds + s * SP;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = xs[ks_slot(r16_wA<H>(e, lane))];
     const ulonglong2* twrow = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
-    if (valid) r16_stages_any<H, true, 1>(x, lane, twrow, hi, A.logm, 0, L, mode);
-    __syncthreads();  // every read of the forward values is done
-    uint64_t* xs = lds + s * SP;
-    if (act) {
-#pragma unroll
-        for (int e = 0; e < 16; ++e) xs[r16_wB<H>(e, lane)] = x[e];
-    }
-    __syncthreads();
-    if (!valid) return;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) x[e] = xs[r16_wA<H>(e, lane)];
     r16_stages_any<H, true, 0>(x, lane, twrow, hi, A.logm, 0, L, mode);
 
     // ---- store (residues stay lazy for the next radix-16 pass)
@@ -505,7 +530,7 @@ static void inner_shape(int H, int D, int G, int* threads, size_t* lds) {
     const int TL = (1 << H) / 16;
     const int rows = std::max(G * D, 2 * G);
     *threads = (rows * TL + 63) / 64 * 64;
-    *lds = (size_t)rows * ((1 << H) + 1) * 8;
+    *lds = (size_t)rows * ((1 << H) + (1 << H) / 16) * 8;  // ks_lds_row<H>() words per row
 }
 
 static bool inner_fits(int H, int D, int G) {
