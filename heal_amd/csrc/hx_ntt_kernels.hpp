@@ -204,6 +204,7 @@ struct PassArgs {
     int32_t scale_out;
     int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 4q) forward, [0, 2q) inverse)
     int32_t lane_major;      // radix-16 passes: a sub-problem's lanes are adjacent threads (rows contiguous in memory)
+    int32_t map_bitrev;      // src_map / dst_map is the bit reversal of logm bits: rows are computed, not loaded
     int32_t top_bitrev;      // radix-16 pass over the low bits (b0 = 0) of rows-contiguous columns whose row map is
                              // the bit reversal: tiles of 32 "top" values with adjacent bit reversals (r16_pass_brt)
     // digit source (src.kind == 3): column axis `gd_axis` is the digit index d,
@@ -690,16 +691,25 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
             }
         } else if (P.b0 == 0 && !P.src_map_alt && INV && es == 8) {
             // inverse row gather of the first pass: the 16 map entries of a lane
-            // form runs of 2^(H-4) >= 4 consecutive rows, read as int4
+            // form runs of 2^(H-4) >= 4 consecutive rows, read as int4 (or
+            // computed when the map is the bit reversal)
             int32_t r[16];
-            const int32_t* mp = P.src_map + jbase + r16_wl<H>(L_IN, lane);
+            if (P.map_bitrev) {
 #pragma unroll
-            for (int e = 0; e < 16; e += 4) {
-                const int4 v4 = *(const int4*)(mp + r16_wc<H>(L_IN, e));
-                r[e] = v4.x;
-                r[e + 1] = v4.y;
-                r[e + 2] = v4.z;
-                r[e + 3] = v4.w;
+                for (int e = 0; e < 16; ++e) {
+                    const int64_t j = jbase + r16_wl<H>(L_IN, lane) + r16_wc<H>(L_IN, e);
+                    r[e] = (int32_t)(__brev((uint32_t)j) >> (32 - P.logm));
+                }
+            } else {
+                const int32_t* mp = P.src_map + jbase + r16_wl<H>(L_IN, lane);
+#pragma unroll
+                for (int e = 0; e < 16; e += 4) {
+                    const int4 v4 = *(const int4*)(mp + r16_wc<H>(L_IN, e));
+                    r[e] = v4.x;
+                    r[e + 1] = v4.y;
+                    r[e + 2] = v4.z;
+                    r[e + 3] = v4.w;
+                }
             }
             const char* colp = (const char*)P.src.ptr + 8 * T.src_off[s];
             const int64_t rs = 8 * P.src.row_stride;
@@ -769,7 +779,8 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const int64_t j = jbase + ((int64_t)r16_w<H>(L_OUT, e, lane) << P.b0);
-        char* ptr = (char*)P.dst.ptr + es * (colbase + (int64_t)map[j] * P.dst.row_stride);
+        const int64_t row = P.map_bitrev ? (int64_t)(__brev((uint32_t)j) >> (32 - P.logm)) : (int64_t)map[j];
+        char* ptr = (char*)P.dst.ptr + es * (colbase + row * P.dst.row_stride);
         if (es == 4) *(int32_t*)ptr = (int32_t)x[e];
         else *(uint64_t*)ptr = x[e];
     }
