@@ -289,7 +289,7 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
         const bool map_side = plan->inverse ? (P.src_map != nullptr && P.src.row_stride == 1)
                                             : (P.dst_map != nullptr && P.dst.row_stride == 1);
         P.top_bitrev = plan->perm_bitrev && rows_contiguous && map_side && P.b0 == 0 && r16_window(H) &&
-                       logm - H >= 5 && npass > 1;
+                       logm - H >= 5 && npass > 1 && P.src.row_stride == 1 && P.dst.row_stride == 1;
         // other passes with the bit-reversal map compute its rows instead of loading them
         P.map_bitrev = plan->perm_bitrev && (P.src_map || P.dst_map);
         launch_pass(P, st);

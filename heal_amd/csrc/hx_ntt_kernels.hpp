@@ -800,7 +800,8 @@ __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
 // forward store, the inverse load) runs with the 32 sub-problems on adjacent
 // threads; the other side with the 2^(H-4) lanes of one sub-problem on adjacent
 // threads (adjacent rows); the LDS exchange between the two rounds switches
-// the thread order.  The column, and so the limb, is the same for the tile.
+// the thread order.  The column, and so the limb, is the same for the tile;
+// rows are adjacent elements on both sides (row stride 1).
 template <int H, bool INV>
 __device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int64_t nblk) {
     constexpr int W = 1 << H;
@@ -839,12 +840,14 @@ __device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int
     uint64_t x[16];
 
     // ---- load: forward rows top * W + w; inverse rows brev(top * W + w)
+    // (rows are adjacent elements: one column pointer for the tile, 32-bit rows)
+    const int64_t* col64 = (const int64_t*)P.src.ptr + src_off;
+    const int32_t* col32 = (const int32_t*)P.src.ptr + src_off;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const int w = r16_w<H>(L_IN, e, lane1);
-        const int64_t row = INV ? ((int64_t)(__brev((uint32_t)w) >> (32 - H)) << ntb) + v0 + s1 : (top1 << H) + w;
-        const int64_t off = src_off + row * P.src.row_stride;
-        const int64_t xv = es_in == 4 ? (int64_t)((const int32_t*)P.src.ptr)[off] : ((const int64_t*)P.src.ptr)[off];
+        const uint32_t w = (uint32_t)r16_w<H>(L_IN, e, lane1);
+        const uint32_t row = INV ? ((__brev(w) >> (32 - H)) << ntb) + (uint32_t)(v0 + s1) : ((uint32_t)top1 << H) + w;
+        const int64_t xv = es_in == 4 ? (int64_t)col32[row] : col64[row];
         uint64_t v = (uint64_t)xv;
         if (P.reduce_in && (xv < 0 || v >= q)) v = residue(limb_modinfo(L), xv);
         x[e] = v;
@@ -861,13 +864,14 @@ __device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int
         for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], q);
     }
     // ---- store: forward rows brev(top * W + w); inverse rows top * W + w
+    uint64_t* out64 = (uint64_t*)P.dst.ptr + dst_off;
+    int32_t* out32 = (int32_t*)P.dst.ptr + dst_off;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const int w = r16_w<H>(L_OUT, e, lane2);
-        const int64_t row = INV ? (top2 << H) + w : ((int64_t)(__brev((uint32_t)w) >> (32 - H)) << ntb) + v0 + s2;
-        const int64_t off = dst_off + row * P.dst.row_stride;
-        if (es_out == 4) ((int32_t*)P.dst.ptr)[off] = (int32_t)x[e];
-        else ((uint64_t*)P.dst.ptr)[off] = x[e];
+        const uint32_t w = (uint32_t)r16_w<H>(L_OUT, e, lane2);
+        const uint32_t row = INV ? ((uint32_t)top2 << H) + w : ((__brev(w) >> (32 - H)) << ntb) + (uint32_t)(v0 + s2);
+        if (es_out == 4) out32[row] = (int32_t)x[e];
+        else out64[row] = x[e];
     }
 }
 
