@@ -155,11 +155,14 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
     const int NF = G * D, NI = 2 * G;
     const int tid = threadIdx.x;
 
+    // instances vary fastest, then limb groups: the workgroups of one XCD share
+    // their twiddles (same limbs and rows) and the partial cache lines at the
+    // edges of neighbouring column groups of the same rows, in one L2
     int64_t tile = xcd_tile(blk, nblk);
-    const int64_t hi = tile % A.n_hi;
-    tile /= A.n_hi;
+    const int64_t b = tile % A.B;
+    tile /= A.B;
     const int64_t tg = tile % A.n_tg;
-    const int64_t b = tile / A.n_tg;
+    const int64_t hi = tile / A.n_tg;
     const int64_t t0 = tg * G;
     const int64_t ncols1 = (int64_t)D * A.k * A.B, ncols2 = 2 * A.k * A.B;
     const int64_t row0 = hi * W;
