@@ -12,12 +12,14 @@
 //       group, the key inner product and digit sum in LDS, and the inverse
 //       pass over the low bits;
 //   K3  inverse pass over the high bits, m^-1 scaling, and the Galois
-//       permutation applied as a per-limb row scatter on store.
+//       permutation applied as a row scatter on store (one shared row map
+//       when every limb has the same permutation).
 // Between the forward and inverse transforms the values stay in the NTT's
-// network order; the key is permuted into that order once
+// network order; the key is permuted into that order once, as residues
 // (hx_keyswitch_prepare_key), which is why the forward and inverse plans must
-// share one permutation.  Every elementwise step uses the reference's C++
-// integer semantics, so the result is bit-identical with the six-call chain.
+// share one permutation.  The digits follow the reference's unsigned digit
+// loop and every other step is exact modular arithmetic on the residues the
+// chain's INTT receives, so the result is bit-identical with the six-call chain.
 #include <map>
 #include <mutex>
 #include <new>
