@@ -162,6 +162,34 @@ HX_HD uint64_t shoup_lazy_nq(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) {
     return ((uint64_t)hi << 32) | (uint32_t)p;
 }
 
+HX_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+// shoup_lazy_nq with a three-product quotient.  With x = xh:xl and
+// wp = ph:pl in 32-bit halves,
+//     floor(x*wp / 2^64) = xh*ph + floor(xh*pl / 2^32) + floor(xl*ph / 2^32) + c,
+// where c in {0, 1, 2} collects the low words of the two cross products and
+// the high word of xl*pl.  Leaving c out gives a quotient at most 2 below
+// Shoup's, so r = x*w - qh*q lies in [0, 4q) for any x < 2^64 and
+// w < q < 2^62 (Shoup's own remainder lies in [0, 2q)).  It saves the
+// fourth partial product and the carry chain of the exact high product.
+HX_HD uint64_t shoup_lazy3_nq(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) {
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+    const uint32_t pl = (uint32_t)wp, ph = (uint32_t)(wp >> 32);
+    const uint64_t qh = (uint64_t)xh * ph + mulhi32(xh, pl) + mulhi32(xl, ph);
+    const uint32_t wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
+    const uint32_t ql = (uint32_t)qh, qhh = (uint32_t)(qh >> 32);
+    const uint32_t nl = (uint32_t)nq, nh = (uint32_t)(nq >> 32);
+    const uint64_t p = (uint64_t)xl * wl + (uint64_t)ql * nl;  // mod 2^64
+    const uint32_t hi = (uint32_t)(p >> 32) + xl * wh + xh * wl + ql * nh + qhh * nl;
+    return ((uint64_t)hi << 32) | (uint32_t)p;
+}
+
 HX_HD uint64_t shoup_mul(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
     uint64_t r = shoup_lazy(x, w, wp, q);
     return r >= q ? r - q : r;

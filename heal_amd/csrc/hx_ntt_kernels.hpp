@@ -202,7 +202,7 @@ struct PassArgs {
     int64_t n_bot_tiles, n_col_tiles;
     int32_t reduce_in;
     int32_t scale_out;
-    int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 4q) forward, [0, 2q) inverse)
+    int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 8q) forward, [0, 4q) inverse)
     int32_t lane_major;      // radix-16 passes: a sub-problem's lanes are adjacent threads (rows contiguous in memory)
     int32_t map_bitrev;      // src_map / dst_map is the bit reversal of logm bits: rows are computed, not loaded
     int32_t top_bitrev;      // radix-16 pass over the low bits (b0 = 0) of rows-contiguous columns whose row map is
@@ -402,15 +402,17 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs P) {
 //
 // Arithmetic per limb (uniform over a sub-problem):
 //   MODE_S32  q < 2^31: 32-bit Shoup, residues kept in [0, q);
-//   MODE_L64  q < 2^62: 64-bit Shoup with Harvey's lazy butterflies
-//             (D. Harvey, "Faster arithmetic for number-theoretic transforms",
-//             J. Symb. Comp. 60, 2014): forward values stay in [0, 4q),
-//             inverse values in [0, 2q), reduced when leaving the transform;
+//   MODE_L64  q < 2^61: Harvey's lazy butterflies (D. Harvey, "Faster
+//             arithmetic for number-theoretic transforms", J. Symb. Comp. 60,
+//             2014) on the three-product Shoup remainder in [0, 4q)
+//             (shoup_lazy3_nq), with every bound doubled: forward values stay
+//             in [0, 8q), inverse values in [0, 4q), reduced when leaving the
+//             transform;
 //   MODE_F64  q < 2^63: 64-bit Shoup, residues kept in [0, q).
 enum : int { MODE_S32 = 0, MODE_L64 = 1, MODE_F64 = 2 };
 
 __device__ __forceinline__ int limb_mode(const LimbInfo& L) {
-    return L.small ? MODE_S32 : (L.q < (uint64_t(1) << 62) ? MODE_L64 : MODE_F64);
+    return L.small ? MODE_S32 : (L.q < (uint64_t(1) << 61) ? MODE_L64 : MODE_F64);
 }
 
 template <int MODE, bool INV>
@@ -429,17 +431,17 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
             x1 = shoup_mul32(d, (uint32_t)w, (uint32_t)wp, q32);
         }
     } else if (MODE == MODE_L64) {
-        const uint64_t q2 = q << 1;
-        if (!INV) {  // x0, x1 in [0, 4q) -> [0, 4q)
-            uint64_t u = x0 >= q2 ? x0 - q2 : x0;
-            uint64_t t = shoup_lazy_nq(x1, w, wp, (uint64_t)0 - q);  // [0, 2q)
+        const uint64_t q4 = q << 2;  // 8q <= 2^64
+        if (!INV) {  // x0, x1 in [0, 8q) -> [0, 8q)
+            uint64_t u = x0 >= q4 ? x0 - q4 : x0;                    // [0, 4q)
+            uint64_t t = shoup_lazy3_nq(x1, w, wp, (uint64_t)0 - q);  // [0, 4q)
             x0 = u + t;
-            x1 = u - t + q2;
-        } else {  // x0, x1 in [0, 2q) -> [0, 2q)
+            x1 = u - t + q4;
+        } else {  // x0, x1 in [0, 4q) -> [0, 4q)
             uint64_t a = x0 + x1;
-            uint64_t d = x0 - x1 + q2;
-            x0 = a >= q2 ? a - q2 : a;
-            x1 = shoup_lazy_nq(d, w, wp, (uint64_t)0 - q);
+            uint64_t d = x0 - x1 + q4;
+            x0 = a >= q4 ? a - q4 : a;
+            x1 = shoup_lazy3_nq(d, w, wp, (uint64_t)0 - q);
         }
     } else {
         if (!INV) {
@@ -455,9 +457,10 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
     }
 }
 
-// Fully reduce a lazy value of a MODE_L64 limb.
+// Fully reduce a lazy value of a MODE_L64 limb (x < 8q).
 __device__ __forceinline__ uint64_t lazy_reduce(uint64_t x, uint64_t q) {
-    const uint64_t q2 = q << 1;
+    const uint64_t q4 = q << 2, q2 = q << 1;
+    x = x >= q4 ? x - q4 : x;
     x = x >= q2 ? x - q2 : x;
     return x >= q ? x - q : x;
 }
@@ -533,7 +536,7 @@ __device__ __forceinline__ void butterfly_last_scaled(uint64_t& x0, uint64_t& x1
         x0 = shoup_mul32(u + v, (uint32_t)L.minv, (uint32_t)L.minv_p, q32);
         x1 = shoup_mul32(u - v + q32, (uint32_t)L.wm, (uint32_t)L.wm_p, q32);
     } else {
-        const uint64_t lift = MODE == MODE_L64 ? q << 1 : q;  // inputs below 2q (lazy) or below q
+        const uint64_t lift = MODE == MODE_L64 ? q << 2 : q;  // inputs below 4q (lazy) or below q
         const uint64_t u = x0, v = x1;
         x0 = shoup_mul(u + v, L.minv, L.minv_p, q);
         x1 = shoup_mul(u - v + lift, L.wm, L.wm_p, q);

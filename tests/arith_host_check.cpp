@@ -70,6 +70,19 @@ int main() {
             CHECK((u128)wp == (((u128)w << 64) / M), "shoup companion");
             uint64_t xx = rng();
             CHECK(hx::shoup_mul(xx, w, wp, M) == (uint64_t)(((u128)xx * w) % M), "shoup_mul");
+            if (M < (1ull << 62)) {
+                // three-product quotient: congruent to x*w, below 4M, for any x < 2^64
+                const uint64_t xs[4] = {xx, ~0ull, (xx | 0xffffffffull), M < (1ull << 61) ? rng() % (8 * M) : rng()};
+                for (uint64_t xv : xs) {
+                    const uint64_t ws[2] = {w, M - 1};
+                    for (uint64_t wv : ws) {
+                        const uint64_t wq = hx::shoup_companion(mi, wv);
+                        const uint64_t r = hx::shoup_lazy3_nq(xv, wv, wq, (uint64_t)0 - M);
+                        CHECK(r < 4 * M && r % M == (uint64_t)(((u128)xv * wv) % M), "shoup_lazy3 x=%llu w=%llu M=%llu",
+                              (unsigned long long)xv, (unsigned long long)wv, (unsigned long long)M);
+                    }
+                }
+            }
         }
         if (M < (1ull << 31) && M > 1) {
             uint32_t q = (uint32_t)M, w = (uint32_t)(rng() % M), xx = (uint32_t)rng();
