@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -185,6 +185,31 @@ def main():
     t_intt = timed(lambda: batched.inv(batched.acc.view(B * 2, N, 1, L), batched.coef), reps, torch)
     intt_limb_gbs = limbs_inv * 2 * N * 8 * reps / t_intt / 1e9
 
+    # HBM rates of the chain's elementwise calls on the batch, each timed alone:
+    # algorithmic bytes = every operand read once and the result written once
+    # (P = one [N, L] int64 polynomial per instance; the Galois table is read once)
+    from heal_amd import capi
+    D, P_ = batched.dnum, 8 * N * L * B
+    dv = batched.dntt.permute(2, 0, 1, 3).unsqueeze(2)
+    coef3, out3 = batched.coef.view(B * 2, N, L), batched.out.view(B * 2, N, L)
+    p_ = batched.prm.p
+    elem_ops = {
+        "g_decomposition": (lambda: capi.g_decomposition(batched.x, batched.digits, D, batched.w), (1 + D) * P_),
+        "modmul_ttt": (lambda: capi.modmul_ttt(dv, batched.evk, p_, batched.prod), 5 * D * P_),
+        "axis_modsum": (lambda: capi.axis_modsum(batched.prod, p_, batched.acc, 0), (2 * D + 2) * P_),
+        "permute": (lambda: capi.permute(coef3, batched.perms, out3, 2, 1, trusted=True),
+                    4 * P_ + batched.perms.numel() * batched.perms.element_size()),
+    }
+    elem_rates = {}
+    for name, (fn, nbytes) in elem_ops.items():
+        fn()
+        t_op = timed(fn, reps, torch)
+        gbs = nbytes * reps / t_op / 1e9
+        elem_rates[name] = {"GBps": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3),
+                            "bytes": nbytes, "ms": round(t_op / reps * 1e3, 4)}
+    torch.cuda.synchronize()
+    assert torch.equal(batched.out, ref_all), "elementwise timing loop changed the chain's output"
+
     result = {
         "metric": "key-switches/sec (N=2^16, L=24, dnum=3, w=60), batched fused form; its hipGraph replay, the "
                   "plain per-call form and NTT limb-GB/s beside",
@@ -207,6 +232,7 @@ def main():
                    "fused_graph_ks_per_s": round(ks_graph, 2),
                    "fused_GBps_at_kernel_boundaries": round(fused_gbs, 1),
                    "ntt_limb_GBps": round(ntt_limb_gbs, 1), "intt_limb_GBps": round(intt_limb_gbs, 1),
+                   "chain_call_hbm": elem_rates,
                    "keyswitch_algorithmic_bytes": batched.algorithmic_bytes() // B,
                    "keyswitch_chain_bytes": batched.chain_bytes() // B,
                    "keyswitch_fused_bytes": batched.fused_bytes() // B,

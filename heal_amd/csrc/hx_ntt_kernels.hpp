@@ -667,9 +667,14 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
             const int shift = wb * d;
             const int rest = wb * P.gd_D;
             unsigned long long over = 0;
+            // all 16 loads are in flight before the first use: the reduction
+            // branch below would otherwise keep the next load behind it
+            int64_t xw[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xw[e] = *(const int64_t*)(base + r16_wc<H>(L_IN, e) * wstride);
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                const int64_t xv = *(const int64_t*)(base + r16_wc<H>(L_IN, e) * wstride);
+                const int64_t xv = xw[e];
                 const uint64_t u = (uint64_t)xv;
                 uint64_t v = (wb && shift < 64) ? ((u >> shift) & mask) : 0;
                 if (d == 0) over += wb ? (rest < 64 && (u >> rest) != 0) : (xv > 0);
