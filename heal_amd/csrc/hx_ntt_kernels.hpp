@@ -851,13 +851,25 @@ __device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int
     // (rows are adjacent elements: one column pointer for the tile, 32-bit rows)
     const int64_t* col64 = (const int64_t*)P.src.ptr + src_off;
     const int32_t* col32 = (const int32_t*)P.src.ptr + src_off;
+    uint32_t rows[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const uint32_t w = (uint32_t)r16_w<H>(L_IN, e, lane1);
-        const uint32_t row = INV ? ((__brev(w) >> (32 - H)) << ntb) + (uint32_t)(v0 + s1) : ((uint32_t)top1 << H) + w;
-        const int64_t xv = es_in == 4 ? (int64_t)col32[row] : col64[row];
-        uint64_t v = (uint64_t)xv;
-        if (P.reduce_in && (xv < 0 || v >= q)) v = residue(limb_modinfo(L), xv);
+        rows[e] = INV ? ((__brev(w) >> (32 - H)) << ntb) + (uint32_t)(v0 + s1) : ((uint32_t)top1 << H) + w;
+    }
+    // all 16 loads are in flight before the first reduction branch
+    int64_t xv[16];
+    if (es_in == 4) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) xv[e] = (int64_t)col32[rows[e]];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) xv[e] = col64[rows[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        uint64_t v = (uint64_t)xv[e];
+        if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
         x[e] = v;
     }
     r16_stages_any<H, INV, L_IN>(x, lane1, twrow, top1, P.logm, P.stage0, L, mode);
