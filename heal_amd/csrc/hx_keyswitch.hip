@@ -60,12 +60,19 @@ struct InnerArgs {
     int64_t n_tg, n_hi;
 };
 
-// LDS rows of K2: one pad slot every 16 (rows of W + W/16), so that 16
-// adjacent lanes of one sub-problem and adjacent sub-problems of a wave both
-// spread over the banks in either register layout.
+// LDS rows of K2 (64-bit slots): one pad slot every 16 and one more per row,
+// W + W/16 + 1 slots.  A 64-bit access hits bank pair (slot mod 16).  Threads
+// with adjacent lanes of one sub-problem spread over the bank pairs through
+// the pad slots; threads with adjacent sub-problems and one lane through the
+// row length, which is 1 mod 16, so that the exchanges with sub-problems on
+// adjacent threads see at most 2-way conflicts (a row length of 0 mod 16 put
+// every sub-problem of a lane on one bank pair).
+__host__ __device__ constexpr int ks_lds_row_of(int H) {
+    return (1 << H) + (1 << H) / 16 + 1;
+}
 template <int H>
 __device__ constexpr int ks_lds_row() {
-    return (1 << H) + (1 << H) / 16;
+    return ks_lds_row_of(H);
 }
 __device__ __forceinline__ int ks_slot(int w) {
     return w + (w >> 4);
@@ -542,7 +549,7 @@ static void inner_shape(int H, int D, int G, int* threads, size_t* lds) {
     const int TL = (1 << H) / 16;
     const int rows = std::max(G * D, 2 * G);
     *threads = (rows * TL + 63) / 64 * 64;
-    *lds = (size_t)rows * ((1 << H) + (1 << H) / 16) * 8;  // ks_lds_row<H>() words per row
+    *lds = (size_t)rows * ks_lds_row_of(H) * 8;
 }
 
 static bool inner_fits(int H, int D, int G) {
