@@ -197,7 +197,7 @@ def main():
         "g_decomposition": (lambda: capi.g_decomposition(batched.x, batched.digits, D, batched.w), (1 + D) * P_),
         "modmul_ttt": (lambda: capi.modmul_ttt(dv, batched.evk, p_, batched.prod), 5 * D * P_),
         "axis_modsum": (lambda: capi.axis_modsum(batched.prod, p_, batched.acc, 0), (2 * D + 2) * P_),
-        "permute": (lambda: capi.permute(coef3, batched.perms, out3, 2, 1, trusted=True),
+        "permute": (lambda: capi.permute(coef3, batched.perms, out3, 2, 1, trusted=True, rows_same=batched.perms_same),
                     4 * P_ + batched.perms.numel() * batched.perms.element_size()),
     }
     elem_rates = {}

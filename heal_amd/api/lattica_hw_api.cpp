@@ -174,7 +174,8 @@ constexpr size_t kMaxPlans = 32;
 static std::mutex g_cache_mu;
 static std::map<PlanKey, std::shared_ptr<PlanHandle>> g_plans;
 static std::list<PlanKey> g_plan_lru;
-static std::set<TensorKey> g_valid_perms;
+// validated perms tables -> every row equals row 0 (the gather then reads row 0 only)
+static std::map<TensorKey, bool> g_valid_perms;
 
 template <typename T>
 std::shared_ptr<PlanHandle> get_plan(int inverse, const std::shared_ptr<DeviceTensor<T>>& p,
@@ -556,19 +557,26 @@ void permute(const TensorPtr<T>& a, const TensorPtr<T>& perms, TensorPtr<T>& res
     require(a, "a"), require(perms, "perms"), require(result, "result");
     hx_view av = HX_V(a), pv = HX_V(perms), rv = HX_V(result);
     const TensorKey key = key_of(perms);  // entries are checked against perms.shape[1]
-    bool known;
+    bool known = false, rows_same = false;
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
-        known = !a->dims.empty() && g_valid_perms.count(key) > 0;
+        auto it = g_valid_perms.find(key);
+        if (!a->dims.empty() && it != g_valid_perms.end()) {
+            known = true;
+            rows_same = it->second;
+        }
     }
     if (known) {
-        check(hx_permute_ex(&av, &pv, &rv, elementwise_axis, perm_axis, HX_PERMS_TRUSTED, t_stream));
+        const uint32_t flags = HX_PERMS_TRUSTED | (rows_same ? HX_PERMS_ROWS_SAME : 0u);
+        check(hx_permute_ex(&av, &pv, &rv, elementwise_axis, perm_axis, flags, t_stream));
     } else {
         check(hx_permute(&av, &pv, &rv, elementwise_axis, perm_axis, t_stream));
         if (!a->dims.empty()) {
+            int same = 0;
+            check(hx_perms_rows_same(&pv, &same, t_stream));
             std::lock_guard<std::mutex> lk(g_cache_mu);
             if (g_valid_perms.size() > 1024) g_valid_perms.clear();
-            g_valid_perms.insert(key);
+            g_valid_perms[key] = same != 0;
         }
     }
     bump(result);

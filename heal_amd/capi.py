@@ -23,6 +23,7 @@ LIB_PATH = os.environ.get("HEAL_AMD_LIB", os.path.join(HERE, "libheal_amd.so"))
 HX_INT32, HX_INT64, HX_FLOAT64 = 0, 1, 2
 HX_OK, HX_ERR_INVALID_ARGUMENT, HX_ERR_OUT_OF_RANGE, HX_ERR_RUNTIME, HX_ERR_BAD_ALLOC = range(5)
 HX_PERMS_TRUSTED = 1
+HX_PERMS_ROWS_SAME = 2
 
 _DTYPE = {torch.int32: HX_INT32, torch.int64: HX_INT64, torch.float64: HX_FLOAT64}
 
@@ -79,6 +80,7 @@ _PROTOS = {
     "hx_intt": [VP, VP, VP, VP, VP, VP, VP, VP, STREAM],
     "hx_permute": [VP, VP, VP, I64, I64, STREAM],
     "hx_permute_ex": [VP, VP, VP, I64, I64, ctypes.c_uint32, STREAM],
+    "hx_perms_rows_same": [VP, ctypes.POINTER(ctypes.c_int), STREAM],
     "hx_ntt_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, VP, VP, VP, VP, STREAM],
     "hx_ntt_plan_exec": [ctypes.c_void_p, VP, VP, STREAM],
     "hx_ntt_plan_destroy": [ctypes.c_void_p],
@@ -297,14 +299,25 @@ def intt(a, p, perm, inv_twiddles, m_inv, out, log2p=None, mu=None):
     return out
 
 
-def permute(a, perms, out, elementwise_axis: int, perm_axis: int, trusted: bool = False):
+def permute(a, perms, out, elementwise_axis: int, perm_axis: int, trusted: bool = False, rows_same: bool = False):
+    """trusted: perms was validated by an earlier call.  rows_same (with trusted):
+    every row of perms equals row 0, as perms_rows_same reports."""
     _need_cuda(a, perms, out)
     if trusted:
+        flags = HX_PERMS_TRUSTED | (HX_PERMS_ROWS_SAME if rows_same else 0)
         check(lib.hx_permute_ex(view(a), view(perms), view(out), int(elementwise_axis), int(perm_axis),
-                                HX_PERMS_TRUSTED, stream_of(out)))
+                                flags, stream_of(out)))
     else:
         check(lib.hx_permute(view(a), view(perms), view(out), int(elementwise_axis), int(perm_axis), stream_of(out)))
     return out
+
+
+def perms_rows_same(perms) -> bool:
+    """True when every row of the 2-D table perms equals its row 0 (synchronous)."""
+    _need_cuda(perms)
+    same = ctypes.c_int(0)
+    check(lib.hx_perms_rows_same(view(perms), ctypes.byref(same), stream_of(perms)))
+    return bool(same.value)
 
 
 def strided_copy(src, dst):

@@ -613,11 +613,55 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermArgs A) {
     if (bad) atomicAdd(&A.status->index_out_of_range, bad);
 }
 
+// Entries of rows 1.. of a 2-D table that differ from the same column of row 0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_perm_rows_diff(const void* perms, int64_t l, int64_t m, int64_t s0,
+                                                           int64_t s1, unsigned long long* count) {
+    const int64_t total = (l - 1) * m;
+    unsigned long long diff = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = 1 + i / m, c = i % m;
+        diff += ld<T>(perms, r * s0 + c * s1) != ld<T>(perms, c * s1);
+    }
+    if (diff) atomicAdd(count, diff);
+}
+
 }  // namespace
 
-// permute (example_impl/permute_impl.cpp:8-65)
+bool perms_rows_same(const hx_view* perms, hipStream_t st) {
+    check_view(perms, "perms");
+    if (perms->dtype != HX_INT32 && perms->dtype != HX_INT64)
+        raise(HX_ERR_INVALID_ARGUMENT, "perms: an int32 or int64 table is expected");
+    if (perms->ndim != 2) raise(HX_ERR_INVALID_ARGUMENT, "Perms must be a 2-D table [l, m].");
+    const int64_t l = perms->dims[0], m = perms->dims[1];
+    if (l <= 1 || m == 0) return true;
+    unsigned long long* count = nullptr;
+    check_hip(hipMallocAsync((void**)&count, sizeof *count, st), "hipMallocAsync(rows count)");
+    unsigned long long h = 0;
+    try {
+        check_hip(hipMemsetAsync(count, 0, sizeof *count, st), "hipMemsetAsync");
+        const unsigned g = grid_for((l - 1) * m, kBlock);
+        if (perms->dtype == HX_INT32)
+            hipLaunchKernelGGL(k_perm_rows_diff<int32_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
+                               perms->strides[0], perms->strides[1], count);
+        else
+            hipLaunchKernelGGL(k_perm_rows_diff<int64_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
+                               perms->strides[0], perms->strides[1], count);
+        check_hip(hipGetLastError(), "rows check launch");
+        check_hip(hipMemcpyAsync(&h, count, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(rows count)");
+        check_hip(hipFreeAsync(count, st), "hipFreeAsync(rows count)");
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        throw;
+    }
+    return h == 0;
+}
+
+// permute (example_impl/permute_impl.cpp:8-65).  rows_same: every row of perms
+// equals row 0 (asserted by the caller), so the gather reads row 0 only.
 void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int64_t ew, int64_t pa, bool trusted,
-             hipStream_t st) {
+             bool rows_same, hipStream_t st) {
     check_view(a, "a");
     check_view(perms, "perms");
     check_view(result, "result");
@@ -672,7 +716,7 @@ void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int6
     A.ew_dim = (int32_t)ew;
     A.pa_dim = (int32_t)pa;
     A.a_pa_stride = src.strides[pa];
-    A.ps0 = perms->strides[0];
+    A.ps0 = rows_same ? 0 : perms->strides[0];
     A.ps1 = perms->strides[1];
     A.m = m;
     A.status = status;
@@ -815,14 +859,22 @@ int hx_g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, 
 int hx_permute(const hx_view* a, const hx_view* perms, const hx_view* result, int64_t elementwise_axis,
                int64_t perm_axis, hx_stream stream) {
     HX_API_BEGIN
-    hx::permute(a, perms, result, elementwise_axis, perm_axis, false, as_stream(stream));
+    hx::permute(a, perms, result, elementwise_axis, perm_axis, false, false, as_stream(stream));
+    HX_API_END
+}
+
+int hx_perms_rows_same(const hx_view* perms, int* same, hx_stream stream) {
+    HX_API_BEGIN
+    if (!same) raise(HX_ERR_INVALID_ARGUMENT, "same pointer must not be null.");
+    *same = hx::perms_rows_same(perms, as_stream(stream)) ? 1 : 0;
     HX_API_END
 }
 
 int hx_permute_ex(const hx_view* a, const hx_view* perms, const hx_view* result, int64_t elementwise_axis,
                   int64_t perm_axis, uint32_t flags, hx_stream stream) {
     HX_API_BEGIN
-    hx::permute(a, perms, result, elementwise_axis, perm_axis, (flags & HX_PERMS_TRUSTED) != 0, as_stream(stream));
+    hx::permute(a, perms, result, elementwise_axis, perm_axis, (flags & HX_PERMS_TRUSTED) != 0,
+                (flags & HX_PERMS_ROWS_SAME) != 0, as_stream(stream));
     HX_API_END
 }
 

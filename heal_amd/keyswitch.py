@@ -32,7 +32,7 @@ class KeySwitchBatch:
         B, N, L, D = batch, n, limbs, dnum
         self.x = P.uniform_residues((B,), self.primes, N, g, self.device)                 # [B, N, L]
         self.evk = P.uniform_residues((D, B, 2), self.primes, N, g, self.device)          # [D, B, 2, N, L]
-        self.perms = P.galois_perms(N, galois_elt, L, domain="coeff", device=self.device)  # [L, N]
+        perms = P.galois_perms(N, galois_elt, L, domain="coeff", device=self.device)  # [L, N]
         dev = self.device
         self.digits = torch.empty((B, N, L, D), dtype=torch.int64, device=dev)
         self.dntt = torch.empty((B, N, D, L), dtype=torch.int64, device=dev)
@@ -42,11 +42,30 @@ class KeySwitchBatch:
         self.out = torch.empty((B, 2, N, L), dtype=torch.int64, device=dev)
         self.fwd = capi.NttPlan(self.prm.p, self.prm.perm, self.prm.twiddles)
         self.inv = capi.NttPlan(self.prm.p, self.prm.perm, self.prm.inv_twiddles, self.prm.m_inv, inverse=True)
-        # validate the Galois map once (synchronous, raises IndexError if out of range)
-        capi.permute(self.coef.view(B * 2, N, L), self.perms, self.out.view(B * 2, N, L), 2, 1)
         self.graph = None
         self.fused = None
         self.key = None
+        self.perms = perms
+
+    # -- the Galois map --------------------------------------------------------
+    @property
+    def perms(self):
+        return self._perms
+
+    @perms.setter
+    def perms(self, perms):
+        """Validates the map once when it is assigned (synchronous; IndexError if
+        an entry is out of range) and records whether every limb has the same
+        row, so that the chain's permute reads row 0 only.  A map edited in place
+        must be assigned again."""
+        B, N, L = self.B, self.n, self.L
+        capi.permute(self.coef.view(B * 2, N, L), perms, self.out.view(B * 2, N, L), 2, 1)
+        self.perms_same = capi.perms_rows_same(perms)
+        self._perms = perms
+        if self.fused is not None:  # the fused plan holds the row map of the old table
+            self.fused.close()
+            self.fused = None
+            self.key = None
 
     # -- fused form: three kernels, key in device layout ----------------------
     def prepare_fused(self):
@@ -71,7 +90,8 @@ class KeySwitchBatch:
         capi.modmul_ttt(dv, self.evk, p, self.prod)
         capi.axis_modsum(self.prod, p, self.acc, 0)
         self.inv(self.acc.view(B * 2, N, 1, L), self.coef)
-        capi.permute(self.coef.view(B * 2, N, L), self.perms, self.out.view(B * 2, N, L), 2, 1, trusted=True)
+        capi.permute(self.coef.view(B * 2, N, L), self.perms, self.out.view(B * 2, N, L), 2, 1, trusted=True,
+                     rows_same=self.perms_same)
         return self.out
 
     # -- hipGraph form --------------------------------------------------------

@@ -170,8 +170,17 @@ int hx_ntt_plan_destroy(hx_ntt_plan* plan);
  * (e.g. by an earlier hx_permute); out-of-range entries still never read
  * outside `a`, they write 0 and count in index_out_of_range. */
 #define HX_PERMS_TRUSTED 1u
+/* HX_PERMS_ROWS_SAME: every row of perms equals its row 0 (as hx_perms_rows_same
+ * reports; a Galois map applied to every limb is such a table), so the gather
+ * reads row 0 for every row: one row of indices stays in cache instead of the
+ * whole [l, m] table being streamed.  The caller asserts it, like
+ * HX_PERMS_TRUSTED; with a table whose rows differ the result is wrong. */
+#define HX_PERMS_ROWS_SAME 2u
 int hx_permute_ex(const hx_view* a, const hx_view* perms, const hx_view* result,
                   int64_t elementwise_axis, int64_t perm_axis, uint32_t flags, hx_stream stream);
+/* *same = 1 when every row of the 2-D table perms equals its row 0, else 0.
+ * Synchronises the stream. */
+int hx_perms_rows_same(const hx_view* perms, int* same, hx_stream stream);
 
 /* ------------------------------------------------- fused hybrid key-switch
  * No counterpart in the reference (whose API composes a key-switch from six

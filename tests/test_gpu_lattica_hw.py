@@ -112,6 +112,36 @@ def test_permute_validation_cache(lhw):
         lhw.permute_64(A, lhw.host_to_device_64(bad), R, 2, 1)
 
 
+def test_permute_cache_identical_rows(lhw):
+    """The validation cache records whether every row equals row 0; the later
+    (trusted) calls then read row 0 only and still equal the oracle.  A table
+    with per-row permutations in the same session keeps reading every row."""
+    g = torch.Generator().manual_seed(5)
+    a = torch.randint(0, 2**60, (2, 64, 6), generator=g, dtype=torch.int64)
+    same = P.galois_perms(64, 5, 6)
+    rows = torch.stack([torch.randperm(64, generator=g) for _ in range(6)]).to(torch.int64)
+    A = lhw.host_to_device_64(a)
+    R = lhw.allocate_on_hardware_64([2, 64, 6])
+    for perms in (same, rows):
+        PM = lhw.host_to_device_64(perms)
+        want = torch.zeros_like(a)
+        assert OracleBackend().permute(a, perms, want, 2, 1) is None
+        for _ in range(3):
+            lhw.permute_64(A, PM, R, 2, 1)
+            assert torch.equal(lhw.device_to_host_64(R), want)
+    # the identical-rows table rewritten in place with per-row maps: a new
+    # version, validated again, read row by row
+    PM = lhw.host_to_device_64(same)
+    lhw.permute_64(A, PM, R, 2, 1)
+    lhw.permute_64(A, PM, R, 2, 1)
+    lhw.copy_from_host_64(PM, rows)
+    want = torch.zeros_like(a)
+    assert OracleBackend().permute(a, rows, want, 2, 1) is None
+    for _ in range(2):
+        lhw.permute_64(A, PM, R, 2, 1)
+        assert torch.equal(lhw.device_to_host_64(R), want)
+
+
 def test_gdecomp_overflow_warning_reported_on_sync(lhw, capfd):
     a = lhw.host_to_device_64(torch.tensor([5, 1 << 40, 7], dtype=torch.int64))
     r = lhw.allocate_on_hardware_64([3, 2])

@@ -161,6 +161,32 @@ def test_permute_random_alias_and_oob(gpu):
     assert GpuBackend().permute(a.cuda(), bad.cuda(), torch.zeros_like(a).cuda(), 2, 1) == "out_of_range"
 
 
+def test_permute_rows_same_flag(gpu):
+    """A table whose rows all equal row 0 (one Galois map for every limb) is
+    reported as such, and the gather that reads row 0 only equals the oracle;
+    tables with one differing entry or per-row permutations are not."""
+    from heal_amd import capi
+
+    g = torch.Generator().manual_seed(17)
+    a = torch.randint(-2**62, 2**62, (4, 512, 24), generator=g, dtype=torch.int64)
+    perms = P.galois_perms(512, 5, 24)
+    want = torch.zeros_like(a)
+    assert OracleBackend().permute(a, perms, want, 2, 1) is None
+    pg = perms.cuda()
+    assert capi.perms_rows_same(pg)
+    out = torch.zeros_like(a).cuda()
+    capi.permute(a.cuda(), pg, out, 2, 1, trusted=True, rows_same=True)
+    assert torch.equal(out.cpu(), want)
+    one_off = perms.clone()
+    one_off[23, 511], one_off[23, 510] = perms[23, 510], perms[23, 511]
+    assert not capi.perms_rows_same(one_off.cuda())
+    rows = torch.stack([torch.randperm(512, generator=g) for _ in range(24)]).to(torch.int64)
+    assert not capi.perms_rows_same(rows.cuda())
+    # a strided (transposed-back) view of the same table and a one-row table
+    assert capi.perms_rows_same(pg.t().contiguous().t())
+    assert capi.perms_rows_same(pg[:1])
+
+
 # ------------------------------------------------------------------- NTT
 def _ntt_case(m, primes, dtype, layout, perm_kind, table_kind, gen, l=2, r=3):
     prm = P.NttParams(m, primes, dtype=dtype)
