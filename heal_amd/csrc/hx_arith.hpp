@@ -190,6 +190,24 @@ HX_HD uint64_t shoup_lazy3_nq(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) 
     return ((uint64_t)hi << 32) | (uint32_t)p;
 }
 
+// Montgomery reduction with R = 2^64 (P. L. Montgomery, "Modular multiplication
+// without trial division", Math. Comp. 44, 1985).  For odd q: qni = -q^-1 mod
+// 2^64 by Newton's iteration x <- x (2 - q x), each step doubling the correct
+// low bits (q x == 1 mod 8 to start).
+HX_HD uint64_t neg_inv64(uint64_t q) {
+    uint64_t x = q;  // q * q == 1 mod 8 for odd q
+    for (int i = 0; i < 5; ++i) x *= 2 - q * x;
+    return (uint64_t)0 - x;
+}
+
+// (hi:lo) * 2^-64 mod q, in [0, 2q), for odd q < 2^63 and (hi:lo) < q * 2^64.
+// With m = lo * qni mod 2^64, lo + m q == 0 mod 2^64, so (hi:lo) + m q is
+// hi + mulhi(m, q) + [lo != 0] times 2^64.
+HX_HD uint64_t redc(uint64_t hi, uint64_t lo, uint64_t q, uint64_t qni) {
+    const uint64_t m = lo * qni;
+    return hi + mulhi64(m, q) + (lo != 0 ? 1u : 0u);
+}
+
 HX_HD uint64_t shoup_mul(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
     uint64_t r = shoup_lazy(x, w, wp, q);
     return r >= q ? r - q : r;

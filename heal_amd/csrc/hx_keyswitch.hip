@@ -85,15 +85,24 @@ __device__ __forceinline__ bool ks_lazy_dot(const LimbInfo& L, int D) {
     return limb_mode(L) == MODE_L64 && (uint64_t)(4 * D) <= ~0ull / L.q;
 }
 
+// Limbs whose prepared key is in Montgomery form, key * 2^64 mod q (odd q of
+// the lazy 64-bit class): the inner product then ends with one Montgomery
+// reduction, whose result in [0, 2q) the lazy inverse butterflies take as is.
+__device__ __forceinline__ bool ks_mont(const LimbInfo& L) {
+    return limb_mode(L) == MODE_L64 && L.qni != 0;
+}
+
 // Key inner product of one lane's 16 values of layout B:
 //     x[e] = (sum_d F_d[w] * key_d[e]) mod q,   w = r16_wB(e, lane),
 // F_d the forward values of digit d in LDS (rows SP apart), key words in
 // [0, q).  One 128-bit accumulation and one reduction when the sum stays below
-// q * 2^64 (`single`), else a reduction per digit.  DC > 0: D = DC at compile
+// q * 2^64 (`single`), else a reduction per digit.  MONT: the key words are
+// key_d * 2^64 mod q and the reduction is Montgomery's with qni = -q^-1 mod 2^64
+// (result in [0, 2q)); otherwise Möller–Granlund (result in [0, q)).  DC > 0: D = DC at compile
 // time, so that all key loads of a lane can be in flight together.
-template <int H, int DC>
+template <int H, int DC, bool MONT>
 __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, const int64_t* kp, int64_t dstride,
-                                       int lane, const ModInfo& mi, int D, bool single) {
+                                       int lane, const ModInfo& mi, int D, bool single, uint64_t qni) {
     constexpr int SP = ks_lds_row<H>();
     constexpr int TL = (1 << H) / 16;
     if (DC > 0) D = DC;
@@ -113,7 +122,8 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
                 u128 acc = 0;
 #pragma unroll
                 for (int d = 0; d < DC; ++d) acc += (u128)xf[d * SP + ks_slot(w)] * kv[d][i];
-                x[e0 + i] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
+                x[e0 + i] = MONT ? redc((uint64_t)(acc >> 64), (uint64_t)acc, mi.M, qni)
+                                 : reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
             }
         }
     } else if (single) {
@@ -125,7 +135,8 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
             for (int d = 0; d < (DC > 0 ? DC : 1); ++d)
                 acc += (u128)xf[d * SP + ks_slot(w)] * (uint64_t)kp[d * dstride + e * TL];
             for (int d = DC > 0 ? DC : 1; d < D; ++d) acc += (u128)xf[d * SP + ks_slot(w)] * (uint64_t)kp[d * dstride + e * TL];
-            x[e] = reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
+            x[e] = MONT ? redc((uint64_t)(acc >> 64), (uint64_t)acc, mi.M, qni)
+                        : reduce128(mi, (uint64_t)(acc >> 64), (uint64_t)acc);
         }
     } else {
         const uint64_t q = mi.M;
@@ -136,7 +147,7 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
                 const uint64_t p = mulmod(mi, reduce64(mi, xf[d * SP + ks_slot(w)]), (uint64_t)kp[d * dstride + e * TL]);
                 s = s >= q - p ? s - (q - p) : s + p;
             }
-            x[e] = s;
+            x[e] = MONT ? redc(0, s, q, qni) : s;
         }
     }
 }
@@ -241,16 +252,27 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
         if (valid) {
             const ModInfo mi = limb_modinfo(L);
             const bool single = ks_lazy_dot(L, D) || (uint64_t)D <= ~0ull / L.q;
+            const uint64_t qni = ks_mont(L) ? L.qni : 0;
             const int64_t kb = A.key_b == 1 ? 0 : b;
             const int64_t* kp = A.key + ((kb * 2 + c) * A.key_k + t) * A.m + row0 + lane;
             const int64_t dstride = A.key_b * 2 * A.key_k * A.m;
             const uint64_t* xf = lds + (int64_t)(D * gi) * SP;
-            switch (D) {
-                case 1: ks_dot<H, 1>(x, xf, kp, dstride, lane, mi, D, single); break;
-                case 2: ks_dot<H, 2>(x, xf, kp, dstride, lane, mi, D, single); break;
-                case 3: ks_dot<H, 3>(x, xf, kp, dstride, lane, mi, D, single); break;
-                case 4: ks_dot<H, 4>(x, xf, kp, dstride, lane, mi, D, single); break;
-                default: ks_dot<H, 0>(x, xf, kp, dstride, lane, mi, D, single); break;
+            if (qni) {
+                switch (D) {
+                    case 1: ks_dot<H, 1, true>(x, xf, kp, dstride, lane, mi, D, single, qni); break;
+                    case 2: ks_dot<H, 2, true>(x, xf, kp, dstride, lane, mi, D, single, qni); break;
+                    case 3: ks_dot<H, 3, true>(x, xf, kp, dstride, lane, mi, D, single, qni); break;
+                    case 4: ks_dot<H, 4, true>(x, xf, kp, dstride, lane, mi, D, single, qni); break;
+                    default: ks_dot<H, 0, true>(x, xf, kp, dstride, lane, mi, D, single, qni); break;
+                }
+            } else {
+                switch (D) {
+                    case 1: ks_dot<H, 1, false>(x, xf, kp, dstride, lane, mi, D, single, 0); break;
+                    case 2: ks_dot<H, 2, false>(x, xf, kp, dstride, lane, mi, D, single, 0); break;
+                    case 3: ks_dot<H, 3, false>(x, xf, kp, dstride, lane, mi, D, single, 0); break;
+                    case 4: ks_dot<H, 4, false>(x, xf, kp, dstride, lane, mi, D, single, 0); break;
+                    default: ks_dot<H, 0, false>(x, xf, kp, dstride, lane, mi, D, single, 0); break;
+                }
             }
             // ---- inverse stages over the low bits (global stages 0 .. hlo-1), layout B
             const ulonglong2* twrow = (const ulonglong2*)(A.tw_i + 2 * t * A.m);
@@ -304,7 +326,7 @@ struct KeyArgs {
     int64_t s[5];  // strides of the [D, B, 2, m, k] view
     int64_t* key;
     const int32_t* rowmap;  // null: identity
-    const LimbInfo* limbs;  // key words are stored as residues mod q_t
+    const LimbInfo* limbs;  // key words are stored as residues mod q_t (Montgomery form where ks_mont)
     int64_t D, B, m, k;
     int32_t H;
 };
@@ -326,7 +348,12 @@ __global__ void k_ks_prepare_key(KeyArgs K) {
         const int64_t jw = (j & ~wmask) | w;
         int64_t u = K.rowmap ? (int64_t)K.rowmap[jw] : jw;
         const int64_t v = K.evk[d * K.s[0] + b * K.s[1] + c * K.s[2] + u * K.s[3] + t * K.s[4]];
-        K.key[e] = (int64_t)residue(limb_modinfo(K.limbs[t]), v);
+        const LimbInfo& L = K.limbs[t];
+        const ModInfo mi = limb_modinfo(L);
+        const uint64_t res = residue(mi, v);
+        // Montgomery form for the limbs whose inner product ends with a
+        // Montgomery reduction: res * (2^64 mod q) mod q
+        K.key[e] = (int64_t)(ks_mont(L) ? mulmod(mi, res, reduce128(mi, 1, 0)) : res);
     }
 }
 

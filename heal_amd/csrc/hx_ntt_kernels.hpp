@@ -21,6 +21,7 @@ struct LimbInfo {
     uint64_t minv_p; // its Shoup companion (32- or 64-bit)
     uint64_t wm;     // inv_twiddles[t, 1] * m^-1 mod q: the last inverse stage with the scaling folded in
     uint64_t wm_p;   // its Shoup companion
+    uint64_t qni;    // -q^-1 mod 2^64 for odd q (Montgomery reduction), 0 for even q
 };
 
 struct hx_ntt_plan {
@@ -105,6 +106,7 @@ __global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, L
     L.v = mi.v;
     L.s = mi.s;
     L.small = q < (int64_t(1) << 31);
+    L.qni = (q & 1) ? neg_inv64((uint64_t)q) : 0;
     if (has_minv) {
         uint64_t mv = residue(mi, ld_elem(minv.ptr, t * minv.s0, minv.elem32));
         L.minv = mv;

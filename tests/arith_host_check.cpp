@@ -84,6 +84,20 @@ int main() {
                 }
             }
         }
+        if ((M & 1) && M > 1 && M < (1ull << 63)) {
+            // Montgomery: redc(T) * 2^64 == T (mod M), redc(T) < 2M, for T < M * 2^64 (2M fits 64 bits)
+            const uint64_t qni = hx::neg_inv64(M);
+            CHECK(M * ((uint64_t)0 - qni) == 1, "neg_inv64 M=%llu", (unsigned long long)M);
+            const uint64_t th[3] = {rng() % M, M - 1, 0};
+            const uint64_t tl[3] = {rng(), ~0ull, rng() % 2};
+            for (int i = 0; i < 3; ++i) {
+                const u128 T = ((u128)th[i] << 64) | tl[i];
+                const uint64_t r = hx::redc(th[i], tl[i], M, qni);
+                CHECK(r < 2 * M, "redc bound M=%llu", (unsigned long long)M);
+                CHECK((((u128)r << 64) % M) == T % M, "redc M=%llu hi=%llu lo=%llu", (unsigned long long)M,
+                      (unsigned long long)th[i], (unsigned long long)tl[i]);
+            }
+        }
         if (M < (1ull << 31) && M > 1) {
             uint32_t q = (uint32_t)M, w = (uint32_t)(rng() % M), xx = (uint32_t)rng();
             uint32_t wp = (uint32_t)(((uint64_t)w << 32) / q);
