@@ -88,9 +88,14 @@ struct IterSpace {
 };
 
 // dims: iteration dims (ndim entries); strides[o][i]: stride of operand o along
-// dim i (0 for broadcast).  keep_last: never merge the last dim away (used when
-// a kernel indexes a table by the last coordinate).
-IterSpace make_iter(int ndim, const int64_t* dims, int nops, const int64_t (*strides)[HX_MAX_DIMS], bool keep_last);
+// dim i (0 for broadcast).  keep_last: never merge the last dim away nor move it
+// (used when a kernel indexes a table by the last coordinate).  The dims are
+// walked in the memory order of operand order_op (its largest stride
+// outermost; ties keep their order), so that a result laid out in another
+// order than its logical one is still written, and read operands with the
+// same layout are still read, in whole lines.
+IterSpace make_iter(int ndim, const int64_t* dims, int nops, const int64_t (*strides)[HX_MAX_DIMS], bool keep_last,
+                    int order_op = 0);
 
 // Decode a linear index into per-operand element offsets; also returns the
 // coordinate along the last iteration dim.

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "hx_internal.hpp"
@@ -89,7 +90,8 @@ bool views_overlap(const hx_view* a, const hx_view* b) {
     return !(ahi < blo || bhi < alo);
 }
 
-IterSpace make_iter(int ndim, const int64_t* dims, int nops, const int64_t (*strides)[HX_MAX_DIMS], bool keep_last) {
+IterSpace make_iter(int ndim, const int64_t* dims, int nops, const int64_t (*strides)[HX_MAX_DIMS], bool keep_last,
+                    int order_op) {
     IterSpace it;
     std::memset(&it, 0, sizeof it);
     int64_t d[HX_MAX_DIMS];
@@ -103,6 +105,18 @@ IterSpace make_iter(int ndim, const int64_t* dims, int nops, const int64_t (*str
         d[n] = dims[i];
         for (int o = 0; o < nops; ++o) s[o][n] = strides[o][i];
         ++n;
+    }
+    // memory order of operand order_op: stable insertion sort of the dims by
+    // decreasing |stride| (the last dim stays in place when keep_last)
+    {
+        const int ns = (keep_last && n > 0) ? n - 1 : n;
+        auto key = [&](int i) { return s[order_op][i] < 0 ? -s[order_op][i] : s[order_op][i]; };
+        for (int i = 1; i < ns; ++i) {
+            for (int j = i; j > 0 && key(j - 1) < key(j); --j) {
+                std::swap(d[j - 1], d[j]);
+                for (int o = 0; o < nops; ++o) std::swap(s[o][j - 1], s[o][j]);
+            }
+        }
     }
     // merge dim j into j+1 (j outer) when every operand is contiguous across them
     int m = 0;
@@ -188,7 +202,7 @@ void strided_copy(const hx_view* src, const hx_view* dst, hipStream_t st) {
         strides[0][i] = src->strides[i];
         strides[1][i] = dst->strides[i];
     }
-    IterSpace it = make_iter(src->ndim, src->dims, 2, strides, false);
+    IterSpace it = make_iter(src->ndim, src->dims, 2, strides, false, 1);  // destination order
     if (it.total == 0) return;
     unsigned g = grid_for(it.total, 256, 8192);
     if (dtype_size(src->dtype) == 4)
