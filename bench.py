@@ -25,7 +25,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the forward pass kernel on the bench workload, keyed by
 # (N, L, B): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
 # counter's reading for a 256 MiB copy (profiles/ntt_pass_hbm_traffic.md)
-NTT_PASS_TRAFFIC = {(65536, 24, 16): 1306934075}
+NTT_PASS_TRAFFIC = {(65536, 24, 16): 1305184556}
 
 
 def parse():
