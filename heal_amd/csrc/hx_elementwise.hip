@@ -535,6 +535,33 @@ __global__ __launch_bounds__(kBlock) void k_gdecomp(GdArgs A) {
     if (over) atomicAdd(&A.status->gdecomp_overflow, over);
 }
 
+// One digit per work-item, the digit index fastest: when the digit axis is
+// contiguous, adjacent work-items write adjacent words, so one store
+// instruction covers whole lines (one work-item per element writes `power`
+// words a digit-stride apart, which splits every store over many lines).  The
+// work-items of one element each load it (a cache hit after the first) and
+// run the reference's digit loop up to their digit: every shift truncates to T.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gdecomp_digits(GdArgs A, FastDiv pw) {
+    const uint64_t mask = (A.base_bits == 0) ? 0 : ((~0ull) >> (64 - A.base_bits));
+    const uint32_t n = (uint32_t)(A.it.total * A.power);
+    unsigned long long over = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        const uint32_t e = fdiv(idx, pw), d = idx - e * A.power;
+        int64_t off[2], last;
+        iter_offsets<2>(A.it, (int64_t)e, off, &last);
+        int64_t v = ld<T>(A.a, off[1]);
+        for (uint32_t i = 0; i < d; ++i) {
+            const uint64_t u = (uint64_t)v;  // size_t arithmetic of the reference
+            v = to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u));
+        }
+        const uint64_t u = (uint64_t)v;
+        st_<T>(A.out, off[0] + d, to_t<T>((int64_t)(u & mask)));
+        if (d + 1 == A.power && to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u)) > 0) ++over;
+    }
+    if (over) atomicAdd(&A.status->gdecomp_overflow, over);
+}
+
 // g_decomposition (example_impl/g_decomposition_impl.cpp:10-67)
 void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hipStream_t st) {
     check_view(a, "a");
@@ -561,9 +588,17 @@ void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, ui
     A.power = (uint32_t)power;
     A.base_bits = (uint32_t)base_bits;
     A.status = device_status();
-    unsigned g = grid_for(A.it.total, kBlock);
-    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_gdecomp<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL(k_gdecomp<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    const int64_t nout = A.it.total * (int64_t)power;
+    if (A.digit_stride == 1 && power >= 1 && nout < (int64_t(1) << 31)) {
+        const FastDiv pw = make_fastdiv((uint32_t)power);
+        unsigned g = grid_for(nout, kBlock);
+        if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_gdecomp_digits<int32_t>, dim3(g), dim3(kBlock), 0, st, A, pw);
+        else hipLaunchKernelGGL(k_gdecomp_digits<int64_t>, dim3(g), dim3(kBlock), 0, st, A, pw);
+    } else {
+        unsigned g = grid_for(A.it.total, kBlock);
+        if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_gdecomp<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+        else hipLaunchKernelGGL(k_gdecomp<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    }
     check_hip(hipGetLastError(), "g_decomposition launch");
 }
 
