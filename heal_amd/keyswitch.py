@@ -36,11 +36,11 @@ class KeySwitchBatch:
         dev = self.device
         self.digits = torch.empty((B, N, L, D), dtype=torch.int64, device=dev)
         self.dntt = torch.empty((B, N, D, L), dtype=torch.int64, device=dev)
-        # products [D, B, 2, N, L] stored with the digit axis inside N (memory
-        # [B, 2, N, D, L]): modmul_ttt walks its result in memory order, so the
-        # digit NTTs it reads ([B, N, D, L]) come in whole lines, and axis_modsum
-        # reads the D terms of each output next to each other
-        self.prod = torch.empty((B, 2, N, D, L), dtype=torch.int64, device=dev).permute(3, 0, 1, 2, 4)
+        # products [D, B, 2, N, L] stored as [B, N, 2, D, L]: modmul_ttt walks its
+        # result in memory order, so the digit NTTs it reads ([B, N, D, L]) come
+        # in whole lines and serve both output polynomials back to back, and
+        # axis_modsum reads the D terms of each output next to each other
+        self.prod = torch.empty((B, N, 2, D, L), dtype=torch.int64, device=dev).permute(3, 0, 2, 1, 4)
         self.acc = torch.empty((B, 2, N, L), dtype=torch.int64, device=dev)
         self.coef = torch.empty((B * 2, N, 1, L), dtype=torch.int64, device=dev)
         self.out = torch.empty((B, 2, N, L), dtype=torch.int64, device=dev)
