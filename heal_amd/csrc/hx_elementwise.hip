@@ -326,7 +326,8 @@ void axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, int6
     }
     AxisArgs A;
     std::memset(&A, 0, sizeof A);
-    A.it = make_iter(result->ndim, result->dims, 2, strides, true);
+    // walked in the memory order of a, which is axis_len times larger than the result
+    A.it = make_iter(result->ndim, result->dims, 2, strides, true, 1);
     if (A.it.total == 0) return;
     A.out = result->data;
     A.a = a->data;
