@@ -2,8 +2,10 @@
 // permute for gfx950.
 //
 // All four are HBM-bound scans: one grid-stride loop per op over the
-// coalesced iteration space of the output, 64-bit element offsets decoded
-// with 32-bit fast division, and exact modular arithmetic from hx_arith.hpp.
+// coalesced iteration space of the output, walked in the memory order of the
+// output (of the input for axis_modsum, which reads axis_len times more than
+// it writes), 64-bit element offsets decoded with 32-bit fast division, and
+// exact modular arithmetic from hx_arith.hpp.
 // A modulus that varies only along the last axis (the RNS limb axis of HEAL
 // tensors) is turned into a table of Möller–Granlund reciprocals in LDS once
 // per workgroup, so the per-element cost is one double-width product and one
