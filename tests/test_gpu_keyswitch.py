@@ -112,6 +112,30 @@ def test_fused_keyswitch_per_limb_permutations(gpu):
         ks.close()
 
 
+def test_fused_keyswitch_mixed_prime_classes(gpu):
+    """Limbs of every butterfly class in one key-switch: 30/31-bit primes (32-bit
+    butterflies, Möller–Granlund inner product), 55/60-bit primes (lazy 64-bit
+    butterflies, key in Montgomery form) and 61/62-bit primes (fully reduced
+    butterflies, Möller–Granlund inner product).  The fused form equals the
+    six-call chain, which equals the oracle's chain."""
+    from heal_amd import params as P
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    primes = P.ntt_primes(4096, [30, 61, 55, 62, 31, 60])
+    print(f"[fused classes] primes={primes}", flush=True)
+    ks = KeySwitchBatch(n=4096, limbs=len(primes), dnum=3, base_bits=24, batch=2, seed=123, primes=primes)
+    ks.run()
+    torch.cuda.synchronize()
+    want = ks.out.clone()
+    out = torch.zeros_like(want)
+    ks.run_fused(out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, want)
+    for b in range(2):
+        assert torch.equal(want[b].cpu(), oracle_keyswitch(ks, b))
+    ks.close()
+
+
 def test_fused_keyswitch_full_size_matches_plain_chain(gpu):
     from heal_amd.keyswitch import KeySwitchBatch
 
