@@ -185,6 +185,16 @@ def test_permute_rows_same_flag(gpu):
     # a strided (transposed-back) view of the same table and a one-row table
     assert capi.perms_rows_same(pg.t().contiguous().t())
     assert capi.perms_rows_same(pg[:1])
+    # int32 tensors and table
+    a32 = torch.randint(-2**31, 2**31, (3, 512, 24), generator=g, dtype=torch.int32)
+    want32 = torch.zeros_like(a32)
+    assert OracleBackend().permute(a32, perms.to(torch.int32), want32, 2, 1) is None
+    pg32 = pg.to(torch.int32)
+    assert capi.perms_rows_same(pg32)
+    assert not capi.perms_rows_same(one_off.to(torch.int32).cuda())
+    out32 = torch.zeros_like(a32).cuda()
+    capi.permute(a32.cuda(), pg32, out32, 2, 1, trusted=True, rows_same=True)
+    assert torch.equal(out32.cpu(), want32)
 
 
 # ------------------------------------------------------------------- NTT
