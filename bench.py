@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-samples", type=int, default=2)
+    ap.add_argument("--cpu-samples", type=int, default=5)  # about 10 s of host work
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (scales HBM byte counters under rocprofv3 --pmc)")
     return ap.parse_args()
