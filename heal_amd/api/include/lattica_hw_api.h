@@ -151,6 +151,22 @@ void synchronize();
 // Number of cached NTT plans / validated permutation tables (for tests).
 size_t plan_cache_size();
 
+// ---- graph capture (no counterpart in the reference)
+// A graph captured from ops keeps addressing the device data of the NTT plans
+// those ops used, so the plans must outlive the graph.  capture_begin() starts
+// a record, for the calling thread, of every plan used while its stream is
+// being captured; capture_end() returns the record, which the caller keeps as
+// long as the graph (dropping it releases the plans; the graph must have
+// finished running).  Plans used by a capture outside such a record are kept
+// until the process ends.  Plan creation and table checks that would
+// synchronise run on a side stream during capture, so tables must be resident
+// before the capture begins.
+struct CapturePins;
+void capture_begin();
+std::shared_ptr<CapturePins> capture_end();
+// Plans currently held by captures (records and process-lifetime pins; for tests).
+size_t captured_plan_count();
+
 }  // namespace lattica_hw_api
 
 #endif  // HEAL_AMD_LATTICA_HW_API_H

@@ -170,7 +170,7 @@ struct PlanHandle {
     }
 };
 
-constexpr size_t kMaxPlans = 32;
+constexpr size_t kMaxPlans = 64;
 static std::mutex g_cache_mu;
 static std::map<PlanKey, std::shared_ptr<PlanHandle>> g_plans;
 static std::list<PlanKey> g_plan_lru;
@@ -212,6 +212,39 @@ void require(const std::shared_ptr<DeviceTensor<T>>& t, const char* name) {
     if (!t) throw std::invalid_argument(std::string(name) + " pointer must not be null.");
 }
 
+// ---- plans held by captured graphs
+static thread_local std::shared_ptr<CapturePins> t_pins;
+static std::mutex g_pin_mu;
+static std::vector<std::shared_ptr<PlanHandle>> g_forever_pins;  // captures outside a record
+static std::atomic<int64_t> g_pinned{0};
+
+bool stream_capturing() {
+    int c = 0;
+    check(hx_stream_capturing(t_stream, &c));
+    return c != 0;
+}
+
+}  // namespace detail
+
+struct CapturePins {
+    std::vector<std::shared_ptr<detail::PlanHandle>> plans;
+    ~CapturePins() { detail::g_pinned -= (int64_t)plans.size(); }
+};
+
+namespace detail {
+
+// A plan used by an op while the stream is being captured stays alive with the graph.
+void pin_if_capturing(const std::shared_ptr<PlanHandle>& plan) {
+    if (!stream_capturing()) return;
+    ++g_pinned;
+    if (t_pins) {
+        t_pins->plans.push_back(plan);
+    } else {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        g_forever_pins.push_back(plan);
+    }
+}
+
 }  // namespace detail
 
 using namespace detail;
@@ -223,6 +256,13 @@ size_t plan_cache_size() {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     return g_plans.size() + g_valid_perms.size();
 }
+void capture_begin() { t_pins = std::make_shared<CapturePins>(); }
+std::shared_ptr<CapturePins> capture_end() {
+    auto p = t_pins ? t_pins : std::make_shared<CapturePins>();
+    t_pins.reset();
+    return p;
+}
+size_t captured_plan_count() { return (size_t)g_pinned.load(); }
 
 // ------------------------------------------------------------------ memory
 
@@ -527,6 +567,7 @@ static void ntt_common(int inverse, const TensorPtr<T>& a, const TensorPtr<T>& p
         throw std::invalid_argument("Tensor 'twiddles' must have shape [k, m].");
     if (inverse) require(minv, "m_inv");
     auto plan = get_plan<T>(inverse, p, perm, tw, inverse ? minv : TensorPtr<T>());
+    pin_if_capturing(plan);
     hx_view av = HX_V(a), rv = HX_V(result);
     check(hx_ntt_plan_exec(plan->plan, &av, &rv, t_stream));
     bump(result);

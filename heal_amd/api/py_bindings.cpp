@@ -7,7 +7,7 @@
 // mod_ct (declared by the reference API but not bound there), the transcript
 // operations the reference runtime names without defining (modneg, abs,
 // modmul_axis_sum, set_const_val, moveaxis, get_slice, pad_single_axis), and
-// backend controls (set_stream, synchronize, plan_cache_size).
+// backend controls (set_stream, synchronize, plan_cache_size, capture records).
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 #include <torch/extension.h>
@@ -108,4 +108,13 @@ PYBIND11_MODULE(lattica_hw, m) {
     m.def("get_stream", []() { return reinterpret_cast<uintptr_t>(get_stream()); });
     m.def("synchronize", &synchronize);
     m.def("plan_cache_size", &plan_cache_size);
+    // graph capture: the record returned by capture_end keeps the plans a
+    // captured graph uses; hold it as long as the graph
+    struct PinRecord {
+        std::shared_ptr<CapturePins> pins;
+    };
+    py::class_<PinRecord>(m, "CapturePins");
+    m.def("capture_begin", &capture_begin);
+    m.def("capture_end", []() { return PinRecord{capture_end()}; });
+    m.def("captured_plan_count", &captured_plan_count);
 }

@@ -57,6 +57,7 @@ _PROTOS = {
     "hx_memcpy_d2h": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, STREAM],
     "hx_memcpy_d2d": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, STREAM],
     "hx_stream_sync": [STREAM],
+    "hx_stream_capturing": [STREAM, ctypes.POINTER(ctypes.c_int)],
     "hx_strided_copy": [VP, VP, STREAM],
     "hx_modmul_ttt": [VP, VP, VP, VP, STREAM],
     "hx_modmul_ttc": [VP, VP, I64, VP, STREAM],
@@ -84,6 +85,8 @@ _PROTOS = {
     "hx_ntt_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, VP, VP, VP, VP, STREAM],
     "hx_ntt_plan_exec": [ctypes.c_void_p, VP, VP, STREAM],
     "hx_ntt_plan_destroy": [ctypes.c_void_p],
+    "hx_ntt_plan_workspace_bytes": [ctypes.c_void_p, VP, VP, ctypes.POINTER(ctypes.c_size_t)],
+    "hx_ntt_plan_exec_ws": [ctypes.c_void_p, VP, VP, ctypes.c_void_p, ctypes.c_size_t, STREAM],
     "hx_keyswitch_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p, VP, U64, U64,
                                  STREAM],
     "hx_keyswitch_key_bytes": [ctypes.c_void_p, I64, ctypes.POINTER(ctypes.c_size_t)],
@@ -326,6 +329,12 @@ def strided_copy(src, dst):
     return dst
 
 
+def stream_capturing(stream: int | None = None) -> bool:
+    c = ctypes.c_int(0)
+    check(lib.hx_stream_capturing(stream if stream is not None else stream_of(), ctypes.byref(c)))
+    return bool(c.value)
+
+
 def status_poll(stream: int | None = None) -> dict:
     st = HxDeviceStatus()
     check(lib.hx_status_poll(stream if stream is not None else stream_of(), ctypes.byref(st)))
@@ -345,6 +354,20 @@ class NttPlan:
     def __call__(self, a, out):
         _need_cuda(a, out)
         check(lib.hx_ntt_plan_exec(self._h, view(a), view(out), stream_of(out)))
+        return out
+
+    def workspace_bytes(self, a, out) -> int:
+        """Workspace that exec_ws needs for these views."""
+        n = ctypes.c_size_t()
+        check(lib.hx_ntt_plan_workspace_bytes(self._h, view(a), view(out), ctypes.byref(n)))
+        return n.value
+
+    def exec_ws(self, a, out, ws):
+        """The transform with a caller's workspace (a device tensor of at least
+        workspace_bytes(a, out) bytes, 256-byte aligned)."""
+        _need_cuda(a, out, ws)
+        check(lib.hx_ntt_plan_exec_ws(self._h, view(a), view(out), ws.data_ptr(), ws.numel() * ws.element_size(),
+                                      stream_of(out)))
         return out
 
     def close(self):

@@ -664,7 +664,58 @@ __global__ __launch_bounds__(kBlock) void k_perm_rows_diff(const void* perms, in
     if (diff) atomicAdd(count, diff);
 }
 
+// entries of an [l, m] table outside [0, m)
+template <typename T>
+__global__ void k_perm_check(const void* perms, int64_t l, int64_t m, int64_t s0, int64_t s1,
+                             unsigned long long* count) {
+    const int64_t total = l * m;
+    unsigned long long bad = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = ld<T>(perms, (i / m) * s0 + (i % m) * s1);
+        bad += v < 0 || v >= m;
+    }
+    if (bad) atomicAdd(count, bad);
+}
+
 }  // namespace
+
+// Counts the entries of a 2-D table with one kernel (rows_diff: entries that
+// differ from row 0; else entries outside [0, m)) and waits for the count.
+// Runs on the side stream while st is being captured.
+static unsigned long long perms_count(const hx_view* perms, bool rows_diff, hipStream_t caller) {
+    const int64_t l = perms->dims[0], m = perms->dims[1];
+    SyncScope sc(caller);
+    hipStream_t st = sc.stream();
+    unsigned long long* count = nullptr;
+    check_hip(hipMallocAsync((void**)&count, 256, st), "hipMallocAsync(table count)");
+    unsigned long long h = 0;
+    hipError_t e = hipSuccess;
+    fill_zero(count, 256, st);
+    const int64_t n = rows_diff ? (l - 1) * m : l * m;
+    const unsigned g = grid_for(n, kBlock);
+    const bool i32 = perms->dtype == HX_INT32;
+    if (rows_diff) {
+        if (i32)
+            hipLaunchKernelGGL(k_perm_rows_diff<int32_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
+                               perms->strides[0], perms->strides[1], count);
+        else
+            hipLaunchKernelGGL(k_perm_rows_diff<int64_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
+                               perms->strides[0], perms->strides[1], count);
+    } else {
+        if (i32)
+            hipLaunchKernelGGL(k_perm_check<int32_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
+                               perms->strides[0], perms->strides[1], count);
+        else
+            hipLaunchKernelGGL(k_perm_check<int64_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
+                               perms->strides[0], perms->strides[1], count);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, count, sizeof h, hipMemcpyDeviceToHost, st);
+    (void)hipFreeAsync(count, st);
+    check_hip(e, "table check");
+    sc.sync();
+    return h;
+}
 
 bool perms_rows_same(const hx_view* perms, hipStream_t st) {
     check_view(perms, "perms");
@@ -673,27 +724,7 @@ bool perms_rows_same(const hx_view* perms, hipStream_t st) {
     if (perms->ndim != 2) raise(HX_ERR_INVALID_ARGUMENT, "Perms must be a 2-D table [l, m].");
     const int64_t l = perms->dims[0], m = perms->dims[1];
     if (l <= 1 || m == 0) return true;
-    unsigned long long* count = nullptr;
-    check_hip(hipMallocAsync((void**)&count, sizeof *count, st), "hipMallocAsync(rows count)");
-    unsigned long long h = 0;
-    try {
-        check_hip(hipMemsetAsync(count, 0, sizeof *count, st), "hipMemsetAsync");
-        const unsigned g = grid_for((l - 1) * m, kBlock);
-        if (perms->dtype == HX_INT32)
-            hipLaunchKernelGGL(k_perm_rows_diff<int32_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
-                               perms->strides[0], perms->strides[1], count);
-        else
-            hipLaunchKernelGGL(k_perm_rows_diff<int64_t>, dim3(g), dim3(kBlock), 0, st, perms->data, l, m,
-                               perms->strides[0], perms->strides[1], count);
-        check_hip(hipGetLastError(), "rows check launch");
-        check_hip(hipMemcpyAsync(&h, count, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(rows count)");
-        check_hip(hipFreeAsync(count, st), "hipFreeAsync(rows count)");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
-    } catch (...) {
-        (void)hipStreamSynchronize(st);
-        throw;
-    }
-    return h == 0;
+    return perms_count(perms, true, st) == 0;
 }
 
 // permute (example_impl/permute_impl.cpp:8-65).  rows_same: every row of perms
@@ -715,6 +746,8 @@ void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int6
         raise(HX_ERR_INVALID_ARGUMENT,
               "Perms must have shape [l, m] where l and m match a.shape at elementwise and perm axes.");
     if (view_numel(a) == 0) return;
+    // the table is checked before anything is written (synchronous)
+    if (!trusted && perms_count(perms, false, st)) raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
     // A result that overlaps the input is computed from a private copy of the input.
     hx_view src = *a;
     void* tmp = nullptr;
@@ -730,10 +763,6 @@ void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int6
         strided_copy(a, &src, st);
     }
     DevStatus* status = device_status();
-    if (!trusted) {
-        // count only this call's out-of-range entries
-        check_hip(hipMemsetAsync(&status->index_out_of_range, 0, sizeof(unsigned long long), st), "hipMemsetAsync");
-    }
     // Iteration space without size-1 dropping or merging, so that the two
     // permutation axes keep their own coordinates.
     PermArgs A;
@@ -763,17 +792,6 @@ void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int6
     else hipLaunchKernelGGL(k_permute<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
     check_hip(hipGetLastError(), "permute launch");
     if (tmp) check_hip(hipFreeAsync(tmp, st), "hipFreeAsync");
-    if (!trusted) {
-        unsigned long long bad = 0;
-        check_hip(hipMemcpyAsync(&bad, &status->index_out_of_range, sizeof bad, hipMemcpyDeviceToHost, st),
-                  "hipMemcpyAsync(status)");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
-        if (bad) {
-            check_hip(hipMemsetAsync(&status->index_out_of_range, 0, sizeof bad, st), "hipMemsetAsync");
-            check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
-            raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
-        }
-    }
 }
 
 }  // namespace hx

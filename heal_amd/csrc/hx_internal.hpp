@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/heal_amd.h"
 #include "hx_arith.hpp"
@@ -141,5 +143,67 @@ inline unsigned grid_for(int64_t total, int block, int64_t cap = 4096) {
 }
 
 inline hipStream_t as_stream(hx_stream s) { return (hipStream_t)s; }
+
+void fill_zero(void* ptr, size_t bytes, hipStream_t st);     // fill kernel (graph-safe)
+void fill_u32(uint32_t* ptr, int64_t n, uint32_t value, hipStream_t st);
+
+// ------------------------------------------------ streams, capture, release
+// True while work enqueued on st is being captured into a graph.
+bool stream_capturing(hipStream_t st);
+
+// Work that needs a host synchronisation (plan preparation, table checks).
+// It runs on the caller's stream, or, when that stream is being captured, on
+// the current device's side stream with the thread's capture mode relaxed, so
+// that the capture stays valid.  Tables read on the side stream must be
+// resident before the capture began.
+class SyncScope {
+  public:
+    explicit SyncScope(hipStream_t st);
+    ~SyncScope();
+    SyncScope(const SyncScope&) = delete;
+    SyncScope& operator=(const SyncScope&) = delete;
+    hipStream_t stream() const { return st_; }
+    void sync();
+
+  private:
+    hipStream_t st_;
+    bool escaped_;
+    int mode_;
+};
+
+// Frees pool memory (hipMallocAsync) once every piece of work marked by
+// `events` has run: the device's reclaim stream waits on the events and frees
+// in its own order, so the host never waits.  Destroys the events.
+void release_after(int device, std::vector<hipEvent_t>& events, const std::vector<void*>& ptrs);
+
+// Workspaces of one plan.  Eager execution gets one workspace per stream,
+// reused and grown in that stream's order, so plans can be shared by threads
+// and streams.  An execution that is being captured into a graph gets a
+// workspace of its own that lives as long as the plan (the graph keeps using
+// it).  A workspace has a `flags` part, zero when created, that its users
+// leave zero after every execution, and a `data` part.
+struct Workspaces {
+    struct Ws {
+        void* flags = nullptr;
+        size_t flag_bytes = 0;
+        void* data = nullptr;
+        size_t data_bytes = 0;
+    };
+    std::mutex mu;  // held from acquire() to note()
+    struct Eager {
+        hipStream_t s;
+        Ws ws;
+        hipEvent_t ev;
+    };
+    std::vector<Eager> eager;
+    std::vector<Ws> captured;
+    int device = 0;
+    // A workspace for work about to be enqueued on st (callers hold mu).
+    Ws acquire(hipStream_t st, size_t flag_bytes, size_t data_bytes);
+    // After that work is enqueued: marks its completion on st (callers hold mu).
+    void note(hipStream_t st);
+    // All memory events of eager use, plus the workspaces themselves, for release_after.
+    void release(std::vector<hipEvent_t>& events, std::vector<void*>& ptrs);
+};
 
 }  // namespace hx

@@ -20,6 +20,10 @@
 // share one permutation.  The digits follow the reference's unsigned digit
 // loop and every other step is exact modular arithmetic on the residues the
 // chain's INTT receives, so the result is bit-identical with the six-call chain.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cstdio>
 #include <map>
 #include <mutex>
 #include <new>
@@ -39,8 +43,8 @@ struct hx_keyswitch_plan {
     void* mem;
     int32_t* galmap;  // [k][m]: output row of coefficient j of limb t
     int gal_shared;   // every limb has the same map (K3 then reads row 0 only)
-    mutable void* ws;
-    mutable size_t ws_bytes;
+    // K1 -> K2 and K2 -> K3 intermediates, per stream (see hx::Workspaces)
+    hx::Workspaces* wss;
 };
 
 namespace hx {
@@ -327,12 +331,14 @@ struct KeyArgs {
     int64_t* key;
     const int32_t* rowmap;  // null: identity
     const LimbInfo* limbs;  // key words are stored as residues mod q_t (Montgomery form where ks_mont)
+    unsigned long long* negatives;  // count of negative key words
     int64_t D, B, m, k;
     int32_t H;
 };
 
 __global__ void k_ks_prepare_key(KeyArgs K) {
     const int64_t n = K.D * K.B * 2 * K.k * K.m;
+    unsigned long long neg = 0;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t j = e % K.m, r = e / K.m;
         int64_t t = r % K.k;
@@ -348,6 +354,7 @@ __global__ void k_ks_prepare_key(KeyArgs K) {
         const int64_t jw = (j & ~wmask) | w;
         int64_t u = K.rowmap ? (int64_t)K.rowmap[jw] : jw;
         const int64_t v = K.evk[d * K.s[0] + b * K.s[1] + c * K.s[2] + u * K.s[3] + t * K.s[4]];
+        neg += v < 0;
         const LimbInfo& L = K.limbs[t];
         const ModInfo mi = limb_modinfo(L);
         const uint64_t res = residue(mi, v);
@@ -355,6 +362,7 @@ __global__ void k_ks_prepare_key(KeyArgs K) {
         // Montgomery reduction: res * (2^64 mod q) mod q
         K.key[e] = (int64_t)(ks_mont(L) ? mulmod(mi, res, reduce128(mi, 1, 0)) : res);
     }
+    if (neg) atomicAdd(K.negatives, neg);
 }
 
 // Galois maps: counts per (limb, row), then the inverse map galmap[t][perms[t, n]] = n.
@@ -388,12 +396,15 @@ __global__ void k_ks_gal_same(const int32_t* galmap, int64_t k, int64_t m, unsig
         if (galmap[e] != galmap[e % m]) atomicAdd(&flags[3], 1ull);
 }
 
-// forward and inverse plans must agree on moduli and permutation
+// forward and inverse plans must agree on moduli and permutation; flags[4]
+// counts limbs where a digit (at most dmax) plus q can exceed 2^63 - 1, where
+// the chain's forward NTT of the digits follows the reference's wrapping sums
 __global__ void k_ks_compare(const LimbInfo* a, const LimbInfo* b, int64_t k, const int32_t* ma, const int32_t* mb,
-                             int64_t m, unsigned long long* flags) {
+                             int64_t m, uint64_t dmax, unsigned long long* flags) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k + m; e += (int64_t)gridDim.x * blockDim.x) {
         if (e < k) {
             if (a[e].q != b[e].q) atomicAdd(&flags[2], 1ull);
+            if (dmax > (~0ull >> 1) - a[e].q) atomicAdd(&flags[4], 1ull);
         } else if (ma && mb && ma[e - k] != mb[e - k]) {
             atomicAdd(&flags[2], 1ull);
         }
@@ -405,7 +416,7 @@ __global__ void k_ks_compare(const LimbInfo* a, const LimbInfo* b, int64_t k, co
 static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st);
 
 hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_plan* inv, const hx_view* perms,
-                                         uint64_t dnum, uint64_t base_bits, hipStream_t st) {
+                                         uint64_t dnum, uint64_t base_bits, hipStream_t caller) {
     if (!fwd || !inv) raise(HX_ERR_INVALID_ARGUMENT, "plan pointers must not be null.");
     if (fwd->inverse || !inv->inverse) raise(HX_ERR_INVALID_ARGUMENT, "need a forward and an inverse NTT plan");
     if (fwd->dtype != HX_INT64 || inv->dtype != HX_INT64)
@@ -427,7 +438,13 @@ hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_pl
     if (perms->dtype != HX_INT32 && perms->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, "perms must be integer");
     if (perms->ndim != 2 || perms->dims[0] != k || perms->dims[1] != m)
         raise(HX_ERR_INVALID_ARGUMENT, "Perms must have shape [l, m] where l and m match a.shape at elementwise and perm axes.");
+    if (fwd->ref_all || inv->ref_all)
+        raise(HX_ERR_INVALID_ARGUMENT,
+              "the fused key-switch needs NTT tables in [0, q) and moduli below 2^62 (use the six-call chain)");
 
+    // preparation synchronises; inside a graph capture it runs on the side stream
+    SyncScope sc(caller);
+    hipStream_t st = sc.stream();
     hx_keyswitch_plan* plan = new hx_keyswitch_plan;
     std::memset(plan, 0, sizeof *plan);
     plan->device = dev;
@@ -442,52 +459,64 @@ hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_pl
     plan->w = (int)base_bits;
     plan->G = 1;
     const size_t map_bytes = ((size_t)k * m * 4 + 255) / 256 * 256;
-    check_hip(hipMalloc(&plan->mem, 2 * map_bytes + 256), "hipMalloc(keyswitch plan)");
+    check_hip(hipMallocAsync(&plan->mem, 2 * map_bytes + 256, st), "hipMallocAsync(keyswitch plan)");
     plan->galmap = (int32_t*)plan->mem;
     int32_t* counts = (int32_t*)((char*)plan->mem + map_bytes);
     unsigned long long* flags = (unsigned long long*)((char*)plan->mem + 2 * map_bytes);
+    plan->wss = new Workspaces;
+    plan->wss->device = dev;
     try {
-        check_hip(hipMemsetAsync(counts, 0, map_bytes + 256, st), "hipMemsetAsync");
+        fill_zero(counts, map_bytes + 256, st);
         PlanSrc ps = plan_src(perms);
         unsigned g = grid_for(k * m, 256, 8192);
+        const uint64_t dmax = base_bits ? (~0ull >> (64 - base_bits)) : 0;
         hipLaunchKernelGGL(k_ks_gal_count, dim3(g), dim3(256), 0, st, ps, k, m, counts, flags);
         hipLaunchKernelGGL(k_ks_gal_map, dim3(g), dim3(256), 0, st, ps, k, m, counts, plan->galmap, flags);
         hipLaunchKernelGGL(k_ks_compare, dim3(grid_for(k + m, 256, 1024)), dim3(256), 0, st, fwd->limbs, inv->limbs, k,
-                           fmap ? fwd->rowmap : nullptr, imap ? inv->rowmap : nullptr, m, flags);
+                           fmap ? fwd->rowmap : nullptr, imap ? inv->rowmap : nullptr, m, dmax, flags);
         hipLaunchKernelGGL(k_ks_gal_same, dim3(g), dim3(256), 0, st, plan->galmap, k, m, flags);
         check_hip(hipGetLastError(), "keyswitch plan launch");
-        unsigned long long h[4];
+        unsigned long long h[5];
         check_hip(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        sc.sync();
         if (h[0]) raise(HX_ERR_OUT_OF_RANGE, "Permutation index out of bounds.");
         if (h[1]) raise(HX_ERR_INVALID_ARGUMENT, "the fused key-switch needs every row of perms to be a permutation");
         if (h[2]) raise(HX_ERR_INVALID_ARGUMENT, "forward and inverse plans differ in moduli or permutation");
+        if (h[4])
+            raise(HX_ERR_INVALID_ARGUMENT,
+                  "base_bits too large for these moduli: a digit plus q can exceed 2^63 - 1 (use the six-call chain)");
         plan->gal_shared = h[3] == 0;
         keyswitch_tune(plan, st);
     } catch (...) {
         (void)hipStreamSynchronize(st);
-        (void)hipFree(plan->mem);
-        if (plan->ws) (void)hipFree(plan->ws);
+        std::vector<hipEvent_t> events;
+        std::vector<void*> ptrs{plan->mem};
+        plan->wss->release(events, ptrs);
+        release_after(dev, events, ptrs);
+        delete plan->wss;
         delete plan;
         throw;
     }
     return plan;
 }
 
+// freed after the work already queued with the plan (stream-ordered, no host wait)
 void keyswitch_plan_destroy(hx_keyswitch_plan* plan) {
     if (!plan) return;
-    (void)hipDeviceSynchronize();
-    (void)hipFree(plan->mem);
-    if (plan->ws) {
-        (void)hipFreeAsync(plan->ws, nullptr);
-        (void)hipDeviceSynchronize();
+    std::vector<hipEvent_t> events;
+    std::vector<void*> ptrs{plan->mem};
+    {
+        std::lock_guard<std::mutex> lk(plan->wss->mu);
+        plan->wss->release(events, ptrs);
     }
+    release_after(plan->device, events, ptrs);
+    delete plan->wss;
     delete plan;
 }
 
 // Inverse of one Galois permutation perms [m] (or [1, m]) for keyswitch_exec's
 // row_map: row_map[perms[n]] = n; perms must be a permutation of [0, m).
-void keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int32_t* row_map, hipStream_t st) {
+void keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int32_t* row_map, hipStream_t caller) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     if (!row_map) raise(HX_ERR_INVALID_ARGUMENT, "row_map pointer must not be null.");
     check_view(perms, "perms");
@@ -505,20 +534,22 @@ void keyswitch_row_map(const hx_keyswitch_plan* plan, const hx_view* perms, int3
         v.strides[0] = 0;
     }
     const size_t cnt_bytes = ((size_t)m * 4 + 255) / 256 * 256;
+    SyncScope sc(caller);  // validation synchronises; inside a capture it runs on the side stream
+    hipStream_t st = sc.stream();
     void* mem = nullptr;
     check_hip(hipMallocAsync(&mem, cnt_bytes + 256, st), "hipMallocAsync(row map)");
     int32_t* counts = (int32_t*)mem;
     unsigned long long* flags = (unsigned long long*)((char*)mem + cnt_bytes);
     unsigned long long h[2] = {0, 0};
     try {
-        check_hip(hipMemsetAsync(mem, 0, cnt_bytes + 256, st), "hipMemsetAsync");
+        fill_zero(mem, cnt_bytes + 256, st);
         PlanSrc ps = plan_src(&v);
         unsigned g = grid_for(m, 256, 8192);
         hipLaunchKernelGGL(k_ks_gal_count, dim3(g), dim3(256), 0, st, ps, (int64_t)1, m, counts, flags);
         hipLaunchKernelGGL(k_ks_gal_map, dim3(g), dim3(256), 0, st, ps, (int64_t)1, m, counts, row_map, flags);
         check_hip(hipGetLastError(), "row map launch");
         check_hip(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        sc.sync();
     } catch (...) {
         (void)hipFreeAsync(mem, st);
         throw;
@@ -540,13 +571,23 @@ size_t keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch) {
     return (size_t)plan->D * (size_t)batch * 2 * (size_t)plan->k * (size_t)plan->m * 8;
 }
 
-void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hipStream_t st) {
+// The key is checked as it is laid out (one synchronisation): the chain's
+// modmul keeps the sign of a negative key word (truncating %), after which its
+// INTT follows the reference's signed network, which the fused kernels do not
+// reproduce; such keys are rejected and belong to the six-call chain.
+void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hipStream_t caller) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(evk, "evk", 5);
     if (!key) raise(HX_ERR_INVALID_ARGUMENT, "key pointer must not be null.");
     if (evk->dims[0] != plan->D || evk->dims[2] != 2 || evk->dims[3] != plan->m || evk->dims[4] != plan->k)
         raise(HX_ERR_INVALID_ARGUMENT, "evk must have shape [dnum, B, 2, m, k]");
+    SyncScope sc(caller);
+    hipStream_t st = sc.stream();
+    unsigned long long* neg = nullptr;
+    check_hip(hipMallocAsync((void**)&neg, 256, st), "hipMallocAsync");
+    fill_zero(neg, 256, st);
     KeyArgs K;
+    K.negatives = neg;
     K.evk = (const int64_t*)evk->data;
     for (int i = 0; i < 5; ++i) K.s[i] = evk->strides[i];
     K.key = (int64_t*)key;
@@ -558,7 +599,16 @@ void keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, vo
     K.k = plan->k;
     K.H = plan->hlo;
     hipLaunchKernelGGL(k_ks_prepare_key, dim3(grid_for(K.D * K.B * 2 * K.k * K.m, 256, 8192)), dim3(256), 0, st, K);
-    check_hip(hipGetLastError(), "prepare key launch");
+    unsigned long long h = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, neg, sizeof h, hipMemcpyDeviceToHost, st);
+    (void)hipFreeAsync(neg, st);
+    check_hip(e, "prepare key launch");
+    sc.sync();
+    if (h)
+        raise(HX_ERR_INVALID_ARGUMENT,
+              "the fused key-switch needs key words >= 0: " + std::to_string(h) +
+                  " negative word(s) (use the six-call chain)");
 }
 
 // The three launches of one key-switch, fully described.
@@ -586,11 +636,18 @@ static bool inner_fits(int H, int D, int G) {
     return threads <= 512 && lds <= 64 * 1024;
 }
 
-// Validates one key-switch, sizes the plan's workspace (outside graph capture)
-// and fills the launch arguments.  Returns false for an empty batch.
+// Workspace of a batch of B: the K1 -> K2 intermediate (D k B columns) and
+// the K2 -> K3 one (2 k B columns), m rows each.
+size_t keyswitch_ws_bytes(const hx_keyswitch_plan* plan, int64_t B) {
+    return (size_t)plan->m * (size_t)((int64_t)plan->D * plan->k * B + 2 * plan->k * B) * 8;
+}
+
+// Validates one key-switch and fills the launch arguments, with the
+// intermediates in ws (keyswitch_ws_bytes of the batch).  Returns false for an
+// empty batch.
 bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
-                     int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
-                     hipStream_t st, KsArgs& K) {
+                     int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, void* ws,
+                     KsArgs& K) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(x, "x", 3);
     require_i64(out, "out", 4);
@@ -614,19 +671,8 @@ bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void
     if (!r16_window(plan->hhi) || !r16_window(plan->hlo))
         raise(HX_ERR_RUNTIME, "the fused key-switch needs the radix-16 pass kernels");
     const int64_t ncols1 = (int64_t)D * k * B, ncols2 = 2 * k * B;
-    const size_t bytes = (size_t)m * (size_t)(ncols1 + ncols2) * 8;
-    if (plan->ws_bytes < bytes) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
-        if (cs != hipStreamCaptureStatusNone)
-            raise(HX_ERR_RUNTIME, "key-switch workspace must be sized by one execution before graph capture");
-        if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync(ks workspace)");
-        plan->ws = nullptr;
-        plan->ws_bytes = 0;
-        check_hip(hipMallocAsync(&plan->ws, bytes, st), "hipMallocAsync(ks workspace)");
-        plan->ws_bytes = bytes;
-    }
-    uint64_t* tmp1 = (uint64_t*)plan->ws;
+    if (!ws) raise(HX_ERR_RUNTIME, "internal: key-switch workspace missing");
+    uint64_t* tmp1 = (uint64_t*)ws;
     uint64_t* tmp2 = tmp1 + (size_t)m * ncols1;
     const hx_ntt_plan* F = plan->fwd;
     const hx_ntt_plan* I = plan->inv;
@@ -741,8 +787,14 @@ static void keyswitch_launch(KsArgs& K, hipStream_t st) {
 
 void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                     int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    require_i64(x, "x", 3);
+    if (x->dims[0] == 0) return;
+    std::lock_guard<std::mutex> lk(plan->wss->mu);
+    const Workspaces::Ws w = plan->wss->acquire(st, 0, keyswitch_ws_bytes(plan, x->dims[0]));
     KsArgs K;
-    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, st, K)) keyswitch_launch(K, st);
+    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, w.data, K)) keyswitch_launch(K, st);
+    plan->wss->note(st);
 }
 
 // ------------------------------------------------------------ autotuning
@@ -750,17 +802,70 @@ void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void*
 // bits) and the K2 window (low bits); per (m, k, dnum): the K2 limb group G.
 // Every choice computes the same result; they differ in speed only.  The split
 // is chosen once per (m, dnum) in a process, so keys prepared by one plan serve
-// every plan of that ring degree.
+// every plan of that ring degree.  Choices are kept in a text file keyed by
+// (GPU architecture, m, k, dnum) — HEAL_AMD_TUNE_CACHE, else
+// $HOME/.cache/heal_amd/keyswitch_tune.txt — so that later processes skip the
+// timing; a file that cannot be read or written is ignored.  One lock covers
+// lookup, timing and recording, so concurrent plan creation never pairs a
+// split with a group tuned for another split.
 static std::mutex g_tune_mu;
 static std::map<std::pair<int64_t, int>, int> g_tuned_hlo;
 static std::map<std::tuple<int64_t, int64_t, int>, int> g_tuned_group;
+static bool g_tune_file_read = false;
+
+static std::string tune_cache_path() {
+    if (const char* e = getenv("HEAL_AMD_TUNE_CACHE")) return e;
+    const char* home = getenv("HOME");
+    return home ? std::string(home) + "/.cache/heal_amd/keyswitch_tune.txt" : std::string();
+}
+
+static std::string gpu_arch(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    const size_t c = a.find(':');
+    return c == std::string::npos ? a : a.substr(0, c);
+}
+
+// Lines "arch m k dnum low_bits limb_group"; entries of other architectures
+// and entries a process has already decided are skipped.  Caller holds g_tune_mu.
+static void tune_file_read(const std::string& arch) {
+    if (g_tune_file_read) return;
+    g_tune_file_read = true;
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char a[64];
+    long long m, k;
+    int D, hlo, G;
+    while (fscanf(f, "%63s %lld %lld %d %d %d", a, &m, &k, &D, &hlo, &G) == 6) {
+        if (arch != a || hlo < 6 || hlo > 8 || G < 1 || G > 8) continue;
+        g_tuned_hlo.emplace(std::make_pair((int64_t)m, D), hlo);
+        // a group is kept only with the split it was timed for
+        if (g_tuned_hlo[{(int64_t)m, D}] == hlo) g_tuned_group.emplace(std::make_tuple((int64_t)m, (int64_t)k, D), G);
+    }
+    fclose(f);
+}
+
+static void tune_file_append(const std::string& arch, int64_t m, int64_t k, int D, int hlo, int G) {
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    // best effort: create the directories on the path
+    for (size_t i = path.find('/', 1); i != std::string::npos; i = path.find('/', i + 1))
+        (void)mkdir(path.substr(0, i).c_str(), 0755);
+    FILE* f = fopen(path.c_str(), "a");
+    if (!f) return;
+    fprintf(f, "%s %lld %lld %d %d %d\n", arch.c_str(), (long long)m, (long long)k, D, hlo, G);
+    fclose(f);
+}
 
 // Fastest of five timed key-switches after one warm-up (the minimum is the
 // least noisy estimate of a kernel sequence's cost on a shared machine).
 static float time_keyswitch(hx_keyswitch_plan* plan, const hx_view* x, void* key, size_t key_bytes, const hx_view* out,
-                            hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+                            void* ws, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     KsArgs K;
-    keyswitch_build(plan, x, key, key_bytes, -1, -1, nullptr, out, st, K);
+    keyswitch_build(plan, x, key, key_bytes, -1, -1, nullptr, out, ws, K);
     keyswitch_launch(K, st);  // warm-up
     float best = 1e30f;
     for (int r = 0; r < 5; ++r) {
@@ -785,13 +890,15 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
     for (int G = 1; G <= 8; G *= 2)
         if (G <= std::max<int64_t>(k, 1)) groups.push_back(G);
     const int default_hlo = logm / 2;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string arch = gpu_arch(plan->device);
+    tune_file_read(arch);
     int hlo = -1, G = -1;
     {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
         auto a = g_tuned_hlo.find({m, D});
-        if (a != g_tuned_hlo.end()) hlo = a->second;
+        if (a != g_tuned_hlo.end() && std::find(hlos.begin(), hlos.end(), a->second) != hlos.end()) hlo = a->second;
         auto b = g_tuned_group.find({m, k, D});
-        if (b != g_tuned_group.end()) G = b->second;
+        if (hlo >= 0 && b != g_tuned_group.end() && inner_fits(hlo, D, b->second)) G = b->second;
     }
     if (hlo >= 0 && G >= 0) {
         plan->hlo = hlo;
@@ -804,11 +911,12 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
     // m * k = 2^16 * 24 that is 8 instances (about 0.9 GB of scratch)
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t(1) << 24) / (m * k)));
     const size_t xb = (size_t)B * m * k * 8, ob = 2 * xb, kb = (size_t)D * B * 2 * k * m * 8;
+    const size_t wb = keyswitch_ws_bytes(plan, B);
     void* mem = nullptr;
-    check_hip(hipMallocAsync(&mem, xb + ob + kb, st), "hipMallocAsync(tuning)");
+    check_hip(hipMallocAsync(&mem, xb + ob + kb + wb, st), "hipMallocAsync(tuning)");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     try {
-        check_hip(hipMemsetAsync(mem, 0, xb + ob + kb, st), "hipMemsetAsync(tuning)");
+        fill_zero(mem, xb + ob + kb, st);
         check_hip(hipEventCreate(&e0), "hipEventCreate");
         check_hip(hipEventCreate(&e1), "hipEventCreate");
         hx_view xv, ov;
@@ -825,6 +933,7 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
         ov.dims[0] = B, ov.dims[1] = 2, ov.dims[2] = m, ov.dims[3] = k;
         ov.strides[0] = 2 * m * k, ov.strides[1] = m * k, ov.strides[2] = k, ov.strides[3] = 1;
         void* key = (char*)mem + xb + ob;
+        void* ws = (char*)mem + xb + ob + kb;
         if (hlo < 0) {
             float best = 1e30f;
             for (int h : hlos) {
@@ -832,20 +941,18 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
                 plan->hhi = logm - h;
                 plan->G = std::max(1, std::min(8, 12 / D));
                 while (plan->G > 1 && !inner_fits(h, D, plan->G)) plan->G /= 2;
-                const float t = time_keyswitch(plan, &xv, key, kb, &ov, st, e0, e1);
+                const float t = time_keyswitch(plan, &xv, key, kb, &ov, ws, st, e0, e1);
                 if (t < best || (t == best && h == default_hlo)) best = t, hlo = h;
             }
         }
         plan->hlo = hlo;
         plan->hhi = logm - hlo;
-        if (G < 0) {
-            float best = 1e30f;
-            for (int g : groups) {
-                if (!inner_fits(hlo, D, g)) continue;
-                plan->G = g;
-                const float t = time_keyswitch(plan, &xv, key, kb, &ov, st, e0, e1);
-                if (t < best) best = t, G = g;
-            }
+        float best = 1e30f;
+        for (int g : groups) {
+            if (!inner_fits(hlo, D, g)) continue;
+            plan->G = g;
+            const float t = time_keyswitch(plan, &xv, key, kb, &ov, ws, st, e0, e1);
+            if (t < best) best = t, G = g;
         }
     } catch (...) {
         (void)hipStreamSynchronize(st);
@@ -857,17 +964,12 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     check_hip(hipFreeAsync(mem, st), "hipFreeAsync(tuning)");
-    // the workspace grew for the scratch batch; release it
-    if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync(ks workspace)");
-    plan->ws = nullptr;
-    plan->ws_bytes = 0;
     check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (G < 0 || !inner_fits(hlo, D, G)) raise(HX_ERR_INVALID_ARGUMENT, "dnum too large for the fused key-switch tile");
     plan->G = G;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned_hlo.emplace(std::make_pair(m, D), hlo);
-    g_tuned_group.emplace(std::make_tuple(m, k, D), G);
-    plan->hlo = g_tuned_hlo[{m, D}];
-    plan->hhi = logm - plan->hlo;
+    g_tuned_hlo[{m, D}] = hlo;
+    g_tuned_group[{m, k, D}] = G;
+    tune_file_append(arch, m, k, D, hlo, G);
 }
 
 // ------------------------------------------------------------ grouped execution
@@ -890,36 +992,48 @@ struct hx_keyswitch_group {
     void* mem;                       // launch arguments and tile offsets
     std::vector<void*> workspaces;   // one per member (stream-ordered allocations)
     std::vector<hx::GroupLaunch> launches;
+    hx::Workspaces uses;             // the streams that ran the group (events only)
 };
 
 namespace hx {
 
 hx_keyswitch_group* keyswitch_group_create(const hx_keyswitch_plan* const* plans, const hx_view* xs,
                                            const void* const* keys, const size_t* key_bytes, const hx_view* outs,
-                                           int count, hipStream_t st) {
+                                           int count, hipStream_t caller) {
     if (count < 1) raise(HX_ERR_INVALID_ARGUMENT, "a group needs at least one key-switch");
     if (!plans || !xs || !keys || !key_bytes || !outs) raise(HX_ERR_INVALID_ARGUMENT, "array pointers must not be null.");
     // every member gets a workspace of its own (owned by the group), so members
     // may share a plan and the plans stay free for other work
+    SyncScope sc(caller);
+    hipStream_t st = sc.stream();
     auto* g = new hx_keyswitch_group;
     g->mem = nullptr;
     std::vector<KsArgs> ks;
+    auto drop = [&]() {
+        (void)hipStreamSynchronize(st);
+        std::vector<hipEvent_t> none;
+        std::vector<void*> ptrs = g->workspaces;
+        ptrs.push_back(g->mem);
+        release_after(g->device, none, ptrs);
+        delete g;
+    };
     try {
         check_hip(hipGetDevice(&g->device), "hipGetDevice");
+        g->uses.device = g->device;
         for (int i = 0; i < count; ++i) {
             if (!plans[i]) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
-            hx_keyswitch_plan own = *plans[i];
-            own.ws = nullptr;
-            own.ws_bytes = 0;
+            require_i64(&xs[i], "x", 3);
+            void* ws = nullptr;
+            if (xs[i].dims[0] > 0) {
+                check_hip(hipMallocAsync(&ws, keyswitch_ws_bytes(plans[i], xs[i].dims[0]), st), "hipMallocAsync(group)");
+                g->workspaces.push_back(ws);
+            }
             KsArgs K;
-            const bool any = keyswitch_build(&own, &xs[i], keys[i], key_bytes[i], -1, -1, nullptr, &outs[i], st, K);
-            if (own.ws) g->workspaces.push_back(own.ws);
+            const bool any = keyswitch_build(plans[i], &xs[i], keys[i], key_bytes[i], -1, -1, nullptr, &outs[i], ws, K);
             if (any) ks.push_back(K);
         }
     } catch (...) {
-        (void)hipStreamSynchronize(st);
-        for (void* w : g->workspaces) (void)hipFree(w);
-        delete g;
+        drop();
         throw;
     }
     // host image: per launch [args...][starts...]
@@ -961,14 +1075,11 @@ hx_keyswitch_group* keyswitch_group_create(const hx_keyswitch_plan* const* plans
         }
     }
     try {
-        check_hip(hipMalloc(&g->mem, std::max<size_t>(image.size(), 256)), "hipMalloc(group)");
+        check_hip(hipMallocAsync(&g->mem, std::max<size_t>(image.size(), 256), st), "hipMallocAsync(group)");
         check_hip(hipMemcpyAsync(g->mem, image.data(), image.size(), hipMemcpyHostToDevice, st), "hipMemcpyAsync(group)");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        sc.sync();
     } catch (...) {
-        (void)hipStreamSynchronize(st);
-        if (g->mem) (void)hipFree(g->mem);
-        for (void* w : g->workspaces) (void)hipFree(w);
-        delete g;
+        drop();
         throw;
     }
     return g;
@@ -979,6 +1090,7 @@ void keyswitch_group_exec(const hx_keyswitch_group* g, hipStream_t st) {
     int dev = 0;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     if (dev != g->device) raise(HX_ERR_INVALID_ARGUMENT, "group belongs to another device");
+    std::lock_guard<std::mutex> lk(const_cast<hx_keyswitch_group*>(g)->uses.mu);
     for (const GroupLaunch& L : g->launches) {
         const char* base = (const char*)g->mem;
         const int64_t* starts = (const int64_t*)(base + L.starts_off);
@@ -993,13 +1105,20 @@ void keyswitch_group_exec(const hx_keyswitch_group* g, hipStream_t st) {
             launch_pass_multi(L.H, L.phase == 3, (const PassArgs*)(base + L.args_off), starts, L.count, L.blocks, st);
         }
     }
+    const_cast<hx_keyswitch_group*>(g)->uses.note(st);
 }
 
+// freed after the work already queued with the group (stream-ordered, no host wait)
 void keyswitch_group_destroy(hx_keyswitch_group* g) {
     if (!g) return;
-    (void)hipDeviceSynchronize();
-    (void)hipFree(g->mem);
-    for (void* w : g->workspaces) (void)hipFree(w);
+    std::vector<hipEvent_t> events;
+    std::vector<void*> ptrs = g->workspaces;
+    ptrs.push_back(g->mem);
+    {
+        std::lock_guard<std::mutex> lk(g->uses.mu);
+        g->uses.release(events, ptrs);
+    }
+    release_after(g->device, events, ptrs);
     delete g;
 }
 

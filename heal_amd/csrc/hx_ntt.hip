@@ -56,9 +56,12 @@ static void validate_plan_inputs(int inverse, const hx_view* p, const hx_view* p
 }
 
 hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm, const hx_view* tw,
-                             const hx_view* minv, hipStream_t st) {
+                             const hx_view* minv, hipStream_t caller) {
     validate_plan_inputs(inverse, p, perm, tw, minv);
     const int64_t k = p->dims[0], m = perm->dims[0];
+    // preparation synchronises once; inside a graph capture it runs on the side stream
+    SyncScope sc(caller);
+    hipStream_t st = sc.stream();
     hx_ntt_plan* plan = new hx_ntt_plan;
     std::memset(plan, 0, sizeof *plan);
     check_hip(hipGetDevice(&plan->device), "hipGetDevice");
@@ -67,32 +70,32 @@ hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm,
     plan->m = m;
     plan->k = k;
     plan->logm = m > 0 ? ilog2_exact(m) : 0;
-    size_t limb_bytes = (size_t)(k ? k : 1) * sizeof(LimbInfo);
+    size_t limb_bytes = ((size_t)(k ? k : 1) * sizeof(LimbInfo) + 255) / 256 * 256;
     size_t tw_bytes = (size_t)(k ? k : 1) * (size_t)(m ? m : 1) * 16;
     size_t map_bytes = ((size_t)(m ? m : 1) * 4 + 255) / 256 * 256;
     size_t cnt_bytes = map_bytes;
     size_t flag_bytes = 256;
     size_t total = limb_bytes + tw_bytes + map_bytes + cnt_bytes + flag_bytes;
-    hipError_t e = hipMalloc(&plan->mem, total);
-    if (e != hipSuccess) {
-        delete plan;
-        check_hip(e, "hipMalloc(plan)");
-    }
+    check_hip(hipMallocAsync(&plan->mem, total, st), "hipMallocAsync(plan)");
     char* base = (char*)plan->mem;
     plan->limbs = (LimbInfo*)base;
     plan->tw = (uint64_t*)(base + limb_bytes);
     plan->rowmap = (int32_t*)(base + limb_bytes + tw_bytes);
     int32_t* counts = (int32_t*)(base + limb_bytes + tw_bytes + map_bytes);
     unsigned long long* flags = (unsigned long long*)(base + limb_bytes + tw_bytes + map_bytes + cnt_bytes);
+    // moduli whose reference sums wrap for reduced values: int64 sums of two
+    // residues exceed 2^63 for q > 2^62; the int32 inverse's u + q - v exceeds
+    // 2^31 for q > 2^30 (the int32 forward sums in int64)
+    const int64_t qmax = p->dtype == HX_INT64 ? (int64_t(1) << 62) : (inverse ? (int64_t(1) << 30) : INT64_MAX);
     try {
-        check_hip(hipMemsetAsync(counts, 0, cnt_bytes + flag_bytes, st), "hipMemsetAsync(plan)");
+        fill_zero(counts, cnt_bytes + flag_bytes, st);
         PlanSrc ps = plan_src(p), ms = minv ? plan_src(minv) : PlanSrc{nullptr, 0, 0, 0};
         if (k > 0) {
             hipLaunchKernelGGL(k_plan_limbs, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st, ps, ms, inverse, k,
-                               plan->limbs, flags);
+                               qmax, plan->limbs, flags);
             if (m > 0)
                 hipLaunchKernelGGL(k_plan_twiddles, dim3(grid_for(k * m, 256, 8192)), dim3(256), 0, st, plan_src(tw), k,
-                                   m, plan->limbs, inverse, plan->tw);
+                                   m, plan->limbs, inverse, plan->tw, flags);
         }
         if (m > 0) {
             unsigned g = grid_for(m, 256, 4096);
@@ -102,9 +105,9 @@ hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm,
                 hipLaunchKernelGGL(k_plan_bitrev, dim3(g), dim3(256), 0, st, plan_src(perm), m, plan->logm, flags);
         }
         check_hip(hipGetLastError(), "plan launch");
-        unsigned long long hflags[5];
+        unsigned long long hflags[7];
         check_hip(hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(flags)");
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        sc.sync();
         if (hflags[1]) raise(HX_ERR_OUT_OF_RANGE, "Index out of bounds.");
         if (hflags[0]) raise(HX_ERR_INVALID_ARGUMENT, "NTT moduli must be greater than 1");
         if (hflags[2] == 0) {
@@ -125,31 +128,87 @@ hx_ntt_plan* ntt_plan_create(int inverse, const hx_view* p, const hx_view* perm,
                                plan->rowmap, 2);
         }
         check_hip(hipGetLastError(), "plan map launch");
+        plan->ref_all = (hflags[5] || hflags[6]) && k > 0 && m > 0;
+        if (plan->ref_all) {
+            // the reference's network reads the tables as given
+            const size_t rb = (size_t)k * m * 8 + (size_t)k * 8;
+            void* raw = nullptr;
+            check_hip(hipMallocAsync(&raw, rb, st), "hipMallocAsync(plan tables)");
+            plan->raw_tw = (int64_t*)raw;
+            plan->raw_minv = inverse ? plan->raw_tw + k * m : nullptr;
+            hipLaunchKernelGGL(k_plan_raw, dim3(grid_for(k * m, 256, 8192)), dim3(256), 0, st, plan_src(tw), ms,
+                               inverse, k, m, plan->raw_tw, plan->raw_minv);
+            check_hip(hipGetLastError(), "plan table launch");
+        }
+        plan->wss = new Workspaces;
+        plan->wss->device = plan->device;
+        // the plan is complete before any stream can use it
+        sc.sync();
     } catch (...) {
         (void)hipStreamSynchronize(st);
-        (void)hipFree(plan->mem);
+        if (plan->raw_tw) (void)hipFreeAsync(plan->raw_tw, st);
+        (void)hipFreeAsync(plan->mem, st);
+        (void)hipStreamSynchronize(st);
+        delete plan->wss;
         delete plan;
         throw;
     }
     return plan;
 }
 
+// The plan's memory is freed after the work already queued with it on any
+// stream (stream-ordered, no host wait).  Graphs captured with the plan must
+// be destroyed first.
 void ntt_plan_destroy(hx_ntt_plan* plan) {
     if (!plan) return;
-    // the plan may still be in use by queued work on any stream
-    (void)hipDeviceSynchronize();
-    (void)hipFree(plan->mem);
-    if (plan->ws) {
-        (void)hipFreeAsync(plan->ws, nullptr);
-        (void)hipDeviceSynchronize();
+    std::vector<hipEvent_t> events;
+    std::vector<void*> ptrs{plan->mem, plan->raw_tw};
+    if (plan->wss) {
+        std::lock_guard<std::mutex> lk(plan->wss->mu);
+        plan->wss->release(events, ptrs);
     }
+    release_after(plan->device, events, ptrs);
+    delete plan->wss;
     delete plan;
 }
 
+// Workspace of one execution: [flags: ncols uint32][T1][T2][fixup scratch]
+//   T1   the intermediate between passes, also the fixup kernel's scratch;
+//   T2   staging of a non-permutation perm (forward gather, inverse scatter);
+//   fixup scratch: kRefBlocks rows of m int64 when there is no T1.
+constexpr int64_t kRefBlocks = 1024;
 
+struct ExecShape {
+    int64_t ncols;
+    std::vector<int> win;
+    bool need_tmp;
+    size_t tmp_bytes;   // T1 (+ T2)
+    size_t data_bytes;  // tmp_bytes + fixup scratch
+    size_t flag_bytes;
+    int64_t ref_blocks;
+};
 
-// Executes a plan on a (and writes result); both [l, m, r, k] views.
-void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hipStream_t st) {
+static ExecShape exec_shape(const hx_ntt_plan* plan, const hx_view* a) {
+    ExecShape E;
+    E.ncols = a->dims[0] * a->dims[2] * a->dims[3];
+    const int64_t m = plan->m;
+    const int logm = plan->logm;
+    if (logm <= 11) {
+        E.win.push_back(logm);
+    } else {
+        int npass = (logm + 7) / 8;
+        for (int i = 0; i < npass; ++i) E.win.push_back(logm / npass + (i < logm % npass ? 1 : 0));
+    }
+    const bool two = plan->perm_kind >= PERM_GATHER_FWD;
+    E.need_tmp = E.win.size() > 1 || two;
+    E.tmp_bytes = E.need_tmp ? (size_t)m * (size_t)E.ncols * 8 * (two ? 2 : 1) : 0;
+    E.ref_blocks = std::min<int64_t>(E.ncols, kRefBlocks);
+    E.data_bytes = E.tmp_bytes + (E.need_tmp ? 0 : (size_t)E.ref_blocks * (size_t)m * 8);
+    E.flag_bytes = ((size_t)E.ncols * 4 + 255) / 256 * 256;
+    return E;
+}
+
+static void validate_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_int_view(a, "a");
     require_int_view(result, "result");
@@ -162,8 +221,15 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
     int dev = 0;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     if (dev != plan->device) raise(HX_ERR_INVALID_ARGUMENT, "plan belongs to another device");
+}
+
+// Executes a plan on a (and writes result); both [l, m, r, k] views.  flags
+// (E.flag_bytes) must be zero on entry and are zero again when the queued work
+// has run; data holds E.data_bytes.
+static void ntt_plan_run(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, const ExecShape& E,
+                         uint32_t* flags, void* data, hipStream_t st) {
     const int64_t m = plan->m;
-    const int64_t ncols = a->dims[0] * a->dims[2] * a->dims[3];
+    const int64_t ncols = E.ncols;
     if (ncols == 0 || m == 0) return;
 
     // column axes (l, r, k) ordered by |stride| in a, innermost first, so that
@@ -195,118 +261,160 @@ void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* res
     const int64_t inner_col_stride = A.cstride[0] < 0 ? -A.cstride[0] : A.cstride[0];
     const bool rows_contiguous = (a->strides[1] == 1) && inner_col_stride != 1;
 
-    // windows: one pass up to 11 bits, else near-equal windows of <= 8 bits
-    // (12..16 bits: two register-radix-16 passes)
-    const int logm = plan->logm;
-    std::vector<int> win;
-    if (logm <= 11) {
-        win.push_back(logm);
-    } else {
-        int npass = (logm + 7) / 8;
-        for (int i = 0; i < npass; ++i) win.push_back(logm / npass + (i < logm % npass ? 1 : 0));
-    }
+    const std::vector<int>& win = E.win;
     const int npass = (int)win.size();
-
-    void* tmp = nullptr;
-    const bool need_tmp = npass > 1 || plan->perm_kind == PERM_GATHER_FWD || plan->perm_kind == PERM_SCATTER_INV;
-    if (need_tmp) {
-        // the plan's workspace, grown outside graph capture and reused afterwards
-        size_t bytes = (size_t)m * (size_t)ncols * 8 * (plan->perm_kind >= PERM_GATHER_FWD ? 2 : 1);
-        if (plan->ws_bytes < bytes) {
-            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-            check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
-            if (cs != hipStreamCaptureStatusNone)
-                raise(HX_ERR_RUNTIME, "NTT plan workspace must be sized by one execution before graph capture");
-            if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync(ntt workspace)");
-            plan->ws = nullptr;
-            plan->ws_bytes = 0;
-            check_hip(hipMallocAsync(&plan->ws, bytes, st), "hipMallocAsync(ntt workspace)");
-            plan->ws_bytes = bytes;
-        }
-        tmp = plan->ws;
-    }
-    void* tmp2 = (need_tmp && plan->perm_kind >= PERM_GATHER_FWD) ? (char*)tmp + (size_t)m * ncols * 8 : nullptr;
+    const int logm = plan->logm;
+    void* tmp = E.need_tmp ? data : nullptr;
+    void* tmp2 = (E.need_tmp && plan->perm_kind >= PERM_GATHER_FWD) ? (char*)tmp + (size_t)m * ncols * 8 : nullptr;
     // the intermediate follows the data: columns innermost for limb-last
     // layouts, rows innermost for limb-major ones (coalesced either way)
     auto tmp_at = [&](void* ptr) { return rows_contiguous ? tmp_addr_rows(ptr, m) : tmp_addr(ptr, ncols); };
-    const ColAddr T1 = need_tmp ? tmp_at(tmp) : ColAddr{};
+    const ColAddr T1 = E.need_tmp ? tmp_at(tmp) : ColAddr{};
 
-    // inverse with a non-permutation perm: stage the scattered input first
-    ColAddr in_src = A;
-    if (plan->perm_kind == PERM_SCATTER_INV) {
-        // x[j] = a[map[j]] or, for rows no perm entry hits, the old result row.
-        // A pass with an empty window (H = 0, b0 = 0) visits every row j once
-        // as its own "top" index and runs no stages.
-        PassArgs G = base;
-        G.src = A;
-        G.alt = R;
-        G.dst = tmp_at(tmp2);
-        G.src_map = plan->rowmap;
-        G.src_map_alt = 1;
-        G.H = 0;
-        G.b0 = 0;
-        G.bot_count = 1;
-        G.stage0 = 0;
-        G.S_cols = 64;
-        G.S_bot = 1;
-        G.reduce_in = 0;
-        G.scale_out = 0;
-        launch_pass(G, st);
-        in_src = tmp_at(tmp2);
-    }
-
-    int done_bits = 0;
-    for (int ip = 0; ip < npass; ++ip) {
-        PassArgs P = base;
-        const int H = win[ip];
-        P.H = H;
-        if (!plan->inverse) {
-            P.stage0 = done_bits;            // forward: top bits first
-            P.b0 = logm - done_bits - H;
-        } else {
-            P.stage0 = done_bits;            // inverse: bottom bits first
-            P.b0 = done_bits;
+    if (plan->ref_all) {
+        fill_u32(flags, ncols, 1u, st);
+    } else {
+        // inverse with a non-permutation perm: stage the scattered input first
+        ColAddr in_src = A;
+        if (plan->perm_kind == PERM_SCATTER_INV) {
+            // x[j] = a[map[j]] or, for rows no perm entry hits, the old result row.
+            // A pass with an empty window (H = 0, b0 = 0) visits every row j once
+            // as its own "top" index and runs no stages.
+            PassArgs G = base;
+            G.src = A;
+            G.alt = R;
+            G.dst = tmp_at(tmp2);
+            G.src_map = plan->rowmap;
+            G.src_map_alt = 1;
+            G.H = 0;
+            G.b0 = 0;
+            G.bot_count = 1;
+            G.stage0 = 0;
+            G.S_cols = 64;
+            G.S_bot = 1;
+            G.reduce_in = 0;
+            G.scale_out = 0;
+            launch_pass(G, st);
+            in_src = tmp_at(tmp2);
         }
-        P.bot_count = int64_t(1) << P.b0;
-        const bool first = ip == 0, last = ip == npass - 1;
-        P.src = first ? in_src : T1;
-        P.dst = last ? R : T1;
-        if (last && !plan->inverse && plan->perm_kind == PERM_GATHER_FWD) P.dst = tmp_at(tmp2);
-        P.reduce_in = first && (plan->inverse || m > 1);
-        P.scale_out = last && plan->inverse;
-        if (first && plan->inverse && plan->perm_kind == PERM_MAP) P.src_map = plan->rowmap;
-        if (last && !plan->inverse && plan->perm_kind == PERM_MAP) P.dst_map = plan->rowmap;
-        // radix-16 passes hand lazy residues to a following radix-16 pass
-        P.lazy_out = !last && r16_window(H) && r16_window(win[ip + 1]);
-        const bool col_tiles = !(rows_contiguous && P.bot_count >= 8);
-        configure_tile(P, col_tiles, npass);
-        // a tile of whole columns whose rows are contiguous: lanes of a sub-problem
-        // take adjacent rows (coalesced), not adjacent columns
-        P.lane_major = col_tiles && rows_contiguous;
-        // the bit-reversal row map of rows-contiguous columns, fused into the
-        // pass over the low bits (the forward's last, the inverse's first): tiles
-        // of tops whose permuted rows are adjacent
-        const bool map_side = plan->inverse ? (P.src_map != nullptr && P.src.row_stride == 1)
-                                            : (P.dst_map != nullptr && P.dst.row_stride == 1);
-        P.top_bitrev = plan->perm_bitrev && rows_contiguous && map_side && P.b0 == 0 && r16_window(H) &&
-                       logm - H >= 5 && npass > 1 && P.src.row_stride == 1 && P.dst.row_stride == 1;
-        // other passes with the bit-reversal map compute its rows instead of loading them
-        P.map_bitrev = plan->perm_bitrev && (P.src_map || P.dst_map);
-        launch_pass(P, st);
-        done_bits += H;
+
+        int done_bits = 0;
+        for (int ip = 0; ip < npass; ++ip) {
+            PassArgs P = base;
+            const int H = win[ip];
+            P.H = H;
+            if (!plan->inverse) {
+                P.stage0 = done_bits;            // forward: top bits first
+                P.b0 = logm - done_bits - H;
+            } else {
+                P.stage0 = done_bits;            // inverse: bottom bits first
+                P.b0 = done_bits;
+            }
+            P.bot_count = int64_t(1) << P.b0;
+            const bool first = ip == 0, last = ip == npass - 1;
+            P.src = first ? in_src : T1;
+            P.dst = last ? R : T1;
+            if (last && !plan->inverse && plan->perm_kind == PERM_GATHER_FWD) P.dst = tmp_at(tmp2);
+            P.reduce_in = first && (plan->inverse || m > 1);
+            P.scale_out = last && plan->inverse;
+            if (first && plan->inverse && plan->perm_kind == PERM_MAP) P.src_map = plan->rowmap;
+            if (last && !plan->inverse && plan->perm_kind == PERM_MAP) P.dst_map = plan->rowmap;
+            // the first pass flags columns whose reference result is not the
+            // residue; the last one leaves them to k_ntt_ref (a gather after it
+            // does the same)
+            P.colflag = flags;
+            P.flag_set = first;
+            P.flag_skip = last && !(!plan->inverse && plan->perm_kind == PERM_GATHER_FWD);
+            P.flag_kind = plan->inverse ? FLAG_INV : (plan->dtype == HX_INT32 ? FLAG_FWD32 : FLAG_FWD64);
+            // radix-16 passes hand lazy residues to a following radix-16 pass
+            P.lazy_out = !last && r16_window(H) && r16_window(win[ip + 1]);
+            const bool col_tiles = !(rows_contiguous && P.bot_count >= 8);
+            configure_tile(P, col_tiles, npass);
+            // a tile of whole columns whose rows are contiguous: lanes of a sub-problem
+            // take adjacent rows (coalesced), not adjacent columns
+            P.lane_major = col_tiles && rows_contiguous;
+            // the bit-reversal row map of rows-contiguous columns, fused into the
+            // pass over the low bits (the forward's last, the inverse's first): tiles
+            // of tops whose permuted rows are adjacent
+            const bool map_side = plan->inverse ? (P.src_map != nullptr && P.src.row_stride == 1)
+                                                : (P.dst_map != nullptr && P.dst.row_stride == 1);
+            P.top_bitrev = plan->perm_bitrev && rows_contiguous && map_side && P.b0 == 0 && r16_window(H) &&
+                           logm - H >= 5 && npass > 1 && P.src.row_stride == 1 && P.dst.row_stride == 1;
+            // other passes with the bit-reversal map compute its rows instead of loading them
+            P.map_bitrev = plan->perm_bitrev && (P.src_map || P.dst_map);
+            launch_pass(P, st);
+            done_bits += H;
+        }
+
+        if (!plan->inverse && plan->perm_kind == PERM_GATHER_FWD) {
+            GatherArgs G;
+            G.src = tmp_at(tmp2);
+            G.dst = R;
+            G.ncols = ncols;
+            for (int i = 0; i < 3; ++i) G.cdim[i] = base.cdim[i];
+            G.map = plan->rowmap;
+            G.m = m;
+            G.colflag = flags;
+            hipLaunchKernelGGL(k_row_gather, dim3(grid_for(m * ncols, 256, 8192)), dim3(256), 0, st, G);
+            check_hip(hipGetLastError(), "row gather launch");
+        }
     }
 
-    if (!plan->inverse && plan->perm_kind == PERM_GATHER_FWD) {
-        GatherArgs G;
-        G.src = tmp_at(tmp2);
-        G.dst = R;
-        G.ncols = ncols;
-        for (int i = 0; i < 3; ++i) G.cdim[i] = base.cdim[i];
-        G.map = plan->rowmap;
-        G.m = m;
-        hipLaunchKernelGGL(k_row_gather, dim3(grid_for(m * ncols, 256, 8192)), dim3(256), 0, st, G);
-        check_hip(hipGetLastError(), "row gather launch");
-    }
+    // the reference's own network on the flagged columns (most launches find none)
+    RefArgs F;
+    std::memset(&F, 0, sizeof F);
+    F.src = A;
+    F.old = plan->perm_kind == PERM_SCATTER_INV ? R : ColAddr{};
+    F.dst = R;
+    F.scratch = (int64_t*)(E.need_tmp ? tmp : data);
+    F.colflag = flags;
+    F.rowmap = plan->rowmap;
+    F.perm_kind = plan->perm_kind;
+    F.inverse = plan->inverse;
+    F.is32 = plan->dtype == HX_INT32;
+    F.logm = logm;
+    F.m = m;
+    F.ncols = ncols;
+    for (int i = 0; i < 3; ++i) F.cdim[i] = base.cdim[i];
+    F.limb_axis = base.limb_axis;
+    F.limbs = plan->limbs;
+    F.tw = plan->tw;
+    F.raw_tw = plan->raw_tw;
+    F.raw_minv = plan->raw_minv;
+    hipLaunchKernelGGL(k_ntt_ref, dim3((unsigned)E.ref_blocks), dim3(256), 0, st, F);
+    check_hip(hipGetLastError(), "reference-semantics launch");
+}
+
+// Executes with the plan's own workspace for st (see hx::Workspaces).
+void ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hipStream_t st) {
+    validate_exec(plan, a, result);
+    const ExecShape E = exec_shape(plan, a);
+    if (E.ncols == 0 || plan->m == 0) return;
+    std::lock_guard<std::mutex> lk(plan->wss->mu);
+    Workspaces::Ws w = plan->wss->acquire(st, E.flag_bytes, E.data_bytes);
+    ntt_plan_run(plan, a, result, E, (uint32_t*)w.flags, w.data, st);
+    plan->wss->note(st);
+}
+
+size_t ntt_workspace_bytes(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result) {
+    validate_exec(plan, a, result);
+    const ExecShape E = exec_shape(plan, a);
+    if (E.ncols == 0 || plan->m == 0) return 0;
+    return E.flag_bytes + E.data_bytes;
+}
+
+// Executes with a caller's workspace of ntt_workspace_bytes(): flags are
+// cleared first, so its contents on entry do not matter.
+void ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, void* ws, size_t bytes,
+                      hipStream_t st) {
+    validate_exec(plan, a, result);
+    const ExecShape E = exec_shape(plan, a);
+    if (E.ncols == 0 || plan->m == 0) return;
+    if (!ws) raise(HX_ERR_INVALID_ARGUMENT, "workspace pointer must not be null.");
+    if ((uintptr_t)ws % 256) raise(HX_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+    if (bytes < E.flag_bytes + E.data_bytes) raise(HX_ERR_INVALID_ARGUMENT, "workspace too small for this execution");
+    fill_zero(ws, E.flag_bytes, st);
+    ntt_plan_run(plan, a, result, E, (uint32_t*)ws, (char*)ws + E.flag_bytes, st);
 }
 
 }  // namespace hx
@@ -339,6 +447,20 @@ int hx_ntt_plan_destroy(hx_ntt_plan* plan) {
     HX_API_END
 }
 
+int hx_ntt_plan_workspace_bytes(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, size_t* bytes) {
+    HX_API_BEGIN
+    if (!bytes) raise(HX_ERR_INVALID_ARGUMENT, "bytes pointer must not be null.");
+    *bytes = ntt_workspace_bytes(plan, a, result);
+    HX_API_END
+}
+
+int hx_ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, void* workspace,
+                        size_t workspace_bytes, hx_stream stream) {
+    HX_API_BEGIN
+    ntt_plan_exec_ws(plan, a, result, workspace, workspace_bytes, as_stream(stream));
+    HX_API_END
+}
+
 static int ntt_once(int inverse, const hx_view* a, const hx_view* p, const hx_view* perm, const hx_view* tw,
                     const hx_view* minv, const hx_view* result, hx_stream stream) {
     HX_API_BEGIN
@@ -362,8 +484,7 @@ static int ntt_once(int inverse, const hx_view* a, const hx_view* p, const hx_vi
         ntt_plan_destroy(plan);
         throw;
     }
-    // the plan's memory must outlive the queued passes
-    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    // freed after the queued passes have run (stream-ordered)
     ntt_plan_destroy(plan);
     HX_API_END
 }

@@ -33,14 +33,19 @@ struct hx_ntt_plan {
     int64_t k;
     int perm_kind;        // PERM_* below
     int perm_bitrev;      // perm is the bit reversal of log m bits (a PERM_MAP that is its own inverse)
-    void* mem;            // single device allocation
-    // Intermediate buffer between passes.  A plan executes on one stream at a
-    // time (like an FFT plan); the workspace grows on first use.
-    mutable void* ws;
-    mutable size_t ws_bytes;
+    // Every column runs the reference-semantics kernel (k_ntt_ref): the tables
+    // hold entries outside [0, q), or a modulus is so large that the
+    // reference's T-width sums wrap even for reduced inputs.
+    int ref_all;
+    void* mem;            // single device allocation (stream-ordered pool)
     LimbInfo* limbs;      // [k]
     uint64_t* tw;         // [k][m][2]: (w mod q, companion)
     int32_t* rowmap;      // [m] (see perm_kind)
+    int64_t* raw_tw;      // [k][m] the table as given (ref_all plans only)
+    int64_t* raw_minv;    // [k] m_inv as given (ref_all inverse plans only)
+    // Intermediates between passes: one workspace per stream for eager work,
+    // one per captured execution (see hx::Workspaces).
+    hx::Workspaces* wss;
 };
 
 namespace hx {
@@ -89,7 +94,9 @@ struct PlanSrc {
     int32_t elem32;
 };
 
-__global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, LimbInfo* limbs,
+// flags[5]: table entries outside [0, q); flags[6]: moduli above qmax (the
+// reference's T-width sums of reduced values would wrap)
+__global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, int64_t qmax, LimbInfo* limbs,
                              unsigned long long* flags) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= k) return;
@@ -100,6 +107,7 @@ __global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, L
         atomicAdd(&flags[0], 1ull);
         q = 2;  // keeps the table well-formed; the plan is rejected on the host
     }
+    if (q > qmax) atomicAdd(&flags[6], 1ull);
     ModInfo mi = make_modinfo((uint64_t)q);
     L.q = (uint64_t)q;
     L.d = mi.d;
@@ -108,20 +116,25 @@ __global__ void k_plan_limbs(PlanSrc p, PlanSrc minv, int has_minv, int64_t k, L
     L.small = q < (int64_t(1) << 31);
     L.qni = (q & 1) ? neg_inv64((uint64_t)q) : 0;
     if (has_minv) {
-        uint64_t mv = residue(mi, ld_elem(minv.ptr, t * minv.s0, minv.elem32));
+        const int64_t raw = ld_elem(minv.ptr, t * minv.s0, minv.elem32);
+        if (raw < 0 || raw >= q) atomicAdd(&flags[5], 1ull);
+        uint64_t mv = residue(mi, raw);
         L.minv = mv;
         L.minv_p = L.small ? divq128(mi, 0, mv << 32) : shoup_companion(mi, mv);
     }
     limbs[t] = L;
 }
 
-__global__ void k_plan_twiddles(PlanSrc tw, int64_t k, int64_t m, LimbInfo* limbs, int inverse, uint64_t* out) {
+__global__ void k_plan_twiddles(PlanSrc tw, int64_t k, int64_t m, LimbInfo* limbs, int inverse, uint64_t* out,
+                                unsigned long long* flags) {
     int64_t n = k * m;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t t = e / m, i = e - t * m;
         const LimbInfo& L = limbs[t];  // one thread per limb writes wm, wm_p; the others read q, d, v, s, small
         ModInfo mi = limb_modinfo(L);
-        uint64_t w = residue(mi, ld_elem(tw.ptr, t * tw.s0 + i * tw.s1, tw.elem32));
+        const int64_t raw = ld_elem(tw.ptr, t * tw.s0 + i * tw.s1, tw.elem32);
+        if (raw < 0 || (uint64_t)raw >= L.q) atomicAdd(&flags[5], 1ull);
+        uint64_t w = residue(mi, raw);
         uint64_t wp = L.small ? divq128(mi, 0, w << 32) : shoup_companion(mi, w);
         out[2 * e] = w;
         out[2 * e + 1] = wp;
@@ -223,7 +236,26 @@ struct PassArgs {
     const LimbInfo* limbs;
     const uint64_t* tw;
     int64_t m;
+    // Columns on which the reference's signed arithmetic gives other
+    // representatives than the residues computed here (see k_ntt_ref):
+    // colflag[col] is set by the first pass (flag_set) when a loaded value lies
+    // above the limit of flag_kind, and the last pass (flag_skip) leaves the
+    // result of a flagged column unwritten.  Null: no flags.
+    uint32_t* colflag;
+    int32_t flag_set;
+    int32_t flag_skip;
+    int32_t flag_kind;       // FLAG_* below
 };
+
+// Input limits above which a column is flagged (values compared as unsigned,
+// so negative values are always above): the forward reference network is
+// canonical for every non-negative input whose sums stay within int64; the
+// inverse one for inputs in [0, q).
+enum : int { FLAG_FWD64 = 0, FLAG_FWD32 = 1, FLAG_INV = 2 };
+
+__device__ __forceinline__ uint64_t flag_limit(int kind, uint64_t q) {
+    return kind == FLAG_INV ? q - 1 : (kind == FLAG_FWD32 ? ~0ull >> 1 : (~0ull >> 1) - q);
+}
 
 __device__ __forceinline__ int64_t col_offset(const ColAddr& c, const int64_t* coord) {
     return coord[0] * c.cstride[0] + coord[1] * c.cstride[1] + coord[2] * c.cstride[2];
@@ -255,10 +287,21 @@ struct TileShared {
     int64_t dst_off[64];
     int64_t alt_off[64];
     int64_t jbase[64];
+    int64_t col[64];
     int32_t limb[64];
     int32_t digit[64];
     int32_t valid[64];
+    int32_t bad[64];   // flag_set: a value of this sub-problem is above the flag limit
+    int32_t skip[64];  // flag_skip: the column was flagged by an earlier pass
 };
+
+// The first pass found a value of column `col` above the flag limit.  In a
+// single pass the sub-problem is the whole column, so its own result is
+// withheld too (bad); stores of the same value from several threads are benign.
+__device__ __forceinline__ void mark_flagged(const PassArgs& P, TileShared& T, int s) {
+    T.bad[s] = 1;
+    P.colflag[T.col[s]] = 1u;
+}
 
 __device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared& T, int64_t blk, int64_t nblk) {
     int64_t tile = xcd_tile(blk, nblk);
@@ -287,6 +330,9 @@ __device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared&
         T.digit[s] = P.src.kind == 3 ? (int32_t)coord[P.gd_axis] : 0;
         T.valid[s] = valid;
         T.jbase[s] = (top << (P.b0 + P.H)) + (valid ? bot : 0);
+        T.col[s] = valid ? col : 0;
+        T.bad[s] = 0;
+        T.skip[s] = (P.colflag && P.flag_skip && valid) ? (int32_t)P.colflag[col] : 0;
     }
     __syncthreads();
 }
@@ -294,8 +340,8 @@ __device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared&
 // Coalesced load of the tile into LDS: consecutive threads take consecutive
 // sub-problems of one window row.  `slot(s, w)` gives the LDS index.
 template <typename Slot>
-__device__ __forceinline__ void tile_load(const PassArgs& P, int S, int W, const TileShared& T, uint64_t* xs,
-                                          Slot slot) {
+__device__ __forceinline__ void tile_load(const PassArgs& P, int S, int W, TileShared& T, uint64_t* xs, Slot slot) {
+    const bool flags = P.colflag && P.flag_set;
     for (int e = threadIdx.x; e < S * W; e += blockDim.x) {
         int s = e % S, w = e / S;
         if (!T.valid[s]) continue;
@@ -313,6 +359,7 @@ __device__ __forceinline__ void tile_load(const PassArgs& P, int S, int W, const
             LimbInfo L = P.limbs[T.limb[s]];
             if (x < 0 || (uint64_t)x >= L.q) v = residue(limb_modinfo(L), x);
         }
+        if (flags && (uint64_t)x > flag_limit(P.flag_kind, P.limbs[T.limb[s]].q)) mark_flagged(P, T, s);
         xs[slot(s, w)] = v;
     }
     __syncthreads();
@@ -324,6 +371,7 @@ __device__ __forceinline__ void tile_store(const PassArgs& P, int S, int W, cons
     for (int e = threadIdx.x; e < S * W; e += blockDim.x) {
         int s = e % S, w = e / S;
         if (!T.valid[s]) continue;
+        if (P.flag_skip && (T.skip[s] || T.bad[s])) continue;
         int64_t j = T.jbase[s] + ((int64_t)w << P.b0);
         int64_t row = P.dst_map ? (int64_t)P.dst_map[j] : j;
         uint64_t v = xs[slot(s, w)];
@@ -684,64 +732,64 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
                 x[e] = v;
             }
             if (over) atomicAdd(P.gd_status, over);
-        } else if (!P.src_map) {
+        } else {
+            // raw values first (all 16 loads in flight), then reduction and flags
             int64_t xv[16];
-            if (es == 8) {
+            if (!P.src_map) {
+                if (es == 8) {
 #pragma unroll
-                for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(base + r16_wc<H>(L_IN, e) * wstride);
+                    for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(base + r16_wc<H>(L_IN, e) * wstride);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) xv[e] = *(const int32_t*)(base + r16_wc<H>(L_IN, e) * wstride);
+                }
+            } else if (P.b0 == 0 && !P.src_map_alt && INV && es == 8) {
+                // inverse row gather of the first pass: the 16 map entries of a lane
+                // form runs of 2^(H-4) >= 4 consecutive rows, read as int4 (or
+                // computed when the map is the bit reversal)
+                int32_t r[16];
+                if (P.map_bitrev) {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int64_t j = jbase + r16_wl<H>(L_IN, lane) + r16_wc<H>(L_IN, e);
+                        r[e] = (int32_t)(__brev((uint32_t)j) >> (32 - P.logm));
+                    }
+                } else {
+                    const int32_t* mp = P.src_map + jbase + r16_wl<H>(L_IN, lane);
+#pragma unroll
+                    for (int e = 0; e < 16; e += 4) {
+                        const int4 v4 = *(const int4*)(mp + r16_wc<H>(L_IN, e));
+                        r[e] = v4.x;
+                        r[e + 1] = v4.y;
+                        r[e + 2] = v4.z;
+                        r[e + 3] = v4.w;
+                    }
+                }
+                const char* colp = (const char*)P.src.ptr + 8 * T.src_off[s];
+                const int64_t rs = 8 * P.src.row_stride;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(colp + (int64_t)r[e] * rs);
             } else {
-#pragma unroll
-                for (int e = 0; e < 16; ++e) xv[e] = *(const int32_t*)(base + r16_wc<H>(L_IN, e) * wstride);
-            }
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                uint64_t v = (uint64_t)xv[e];
-                if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
-                x[e] = v;
-            }
-        } else if (P.b0 == 0 && !P.src_map_alt && INV && es == 8) {
-            // inverse row gather of the first pass: the 16 map entries of a lane
-            // form runs of 2^(H-4) >= 4 consecutive rows, read as int4 (or
-            // computed when the map is the bit reversal)
-            int32_t r[16];
-            if (P.map_bitrev) {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
-                    const int64_t j = jbase + r16_wl<H>(L_IN, lane) + r16_wc<H>(L_IN, e);
-                    r[e] = (int32_t)(__brev((uint32_t)j) >> (32 - P.logm));
-                }
-            } else {
-                const int32_t* mp = P.src_map + jbase + r16_wl<H>(L_IN, lane);
-#pragma unroll
-                for (int e = 0; e < 16; e += 4) {
-                    const int4 v4 = *(const int4*)(mp + r16_wc<H>(L_IN, e));
-                    r[e] = v4.x;
-                    r[e + 1] = v4.y;
-                    r[e + 2] = v4.z;
-                    r[e + 3] = v4.w;
+                    const int64_t j = jbase + ((int64_t)r16_w<H>(L_IN, e, lane) << P.b0);
+                    const int32_t r = P.src_map[j];
+                    xv[e] = r >= 0 ? load_any(P.src, T.src_off[s] + (int64_t)r * P.src.row_stride)
+                                   : load_any(P.alt, T.alt_off[s] + j * P.alt.row_stride);
                 }
             }
-            const char* colp = (const char*)P.src.ptr + 8 * T.src_off[s];
-            const int64_t rs = 8 * P.src.row_stride;
-            int64_t xv[16];
-#pragma unroll
-            for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(colp + (int64_t)r[e] * rs);
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 uint64_t v = (uint64_t)xv[e];
                 if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
                 x[e] = v;
             }
-        } else {
+            if (P.colflag && P.flag_set) {
+                const uint64_t lim = flag_limit(P.flag_kind, q);
+                bool bad = false;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int64_t j = jbase + ((int64_t)r16_w<H>(L_IN, e, lane) << P.b0);
-                const int32_t r = P.src_map[j];
-                const int64_t xv = r >= 0 ? load_any(P.src, T.src_off[s] + (int64_t)r * P.src.row_stride)
-                                          : load_any(P.alt, T.alt_off[s] + j * P.alt.row_stride);
-                uint64_t v = (uint64_t)xv;
-                if (P.reduce_in && (xv < 0 || v >= q)) v = residue(limb_modinfo(L), xv);
-                x[e] = v;
+                for (int e = 0; e < 16; ++e) bad |= (uint64_t)xv[e] > lim;
+                if (bad) mark_flagged(P, T, s);
             }
         }
     } else {
@@ -768,6 +816,8 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
 #pragma unroll
         for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], q);
     }
+    // a flagged column is written by the reference-semantics kernel instead
+    if (P.flag_skip && (T.skip[s] || T.bad[s])) return;
 
     // ---- store from registers
     const int es = P.dst.kind == 1 ? 4 : 8;
@@ -848,6 +898,9 @@ __device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int
     const int64_t top2 = (int64_t)(__brev((uint32_t)(v0 + s2)) >> (32 - ntb));
     const int es_in = P.src.kind == 1 ? 4 : 8, es_out = P.dst.kind == 1 ? 4 : 8;
     uint64_t x[16];
+    // a column flagged by the first pass is written by the reference-semantics
+    // kernel (uniform over the workgroup: the tile is one column)
+    if (P.flag_skip && P.colflag[col]) return;
 
     // ---- load: forward rows top * W + w; inverse rows brev(top * W + w)
     // (rows are adjacent elements: one column pointer for the tile, 32-bit rows)
@@ -873,6 +926,13 @@ __device__ __forceinline__ void r16_pass_brt(const PassArgs& P, int64_t blk, int
         uint64_t v = (uint64_t)xv[e];
         if (P.reduce_in && (xv[e] < 0 || v >= q)) v = residue(limb_modinfo(L), xv[e]);
         x[e] = v;
+    }
+    if (P.colflag && P.flag_set) {
+        const uint64_t lim = flag_limit(P.flag_kind, q);
+        bool bad = false;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) bad |= (uint64_t)xv[e] > lim;
+        if (bad) P.colflag[col] = 1u;
     }
     r16_stages_any<H, INV, L_IN>(x, lane1, twrow, top1, P.logm, P.stage0, L, mode);
 #pragma unroll
@@ -930,12 +990,14 @@ struct GatherArgs {
     int64_t cdim[3];
     const int32_t* map;
     int64_t m;
+    const uint32_t* colflag;  // flagged columns are left to the reference-semantics kernel
 };
 
 __global__ void k_row_gather(GatherArgs G) {
     int64_t n = G.m * G.ncols;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t u = e / G.ncols, col = e - u * G.ncols;
+        if (G.colflag && G.colflag[col]) continue;
         int64_t c = col, coord[3];
         coord[0] = c % G.cdim[0];
         c /= G.cdim[0];
@@ -946,6 +1008,141 @@ __global__ void k_row_gather(GatherArgs G) {
         int64_t dof = G.dst.kind == 2 ? col_offset(G.dst, lc) : col_offset(G.dst, coord);
         int64_t v = load_any(G.src, so + (int64_t)G.map[u] * G.src.row_stride);
         store_any(G.dst, dof + u * G.dst.row_stride, (uint64_t)v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reference semantics for flagged columns.
+//
+// The reference runs its butterflies on signed values with C++'s truncating
+// `%` (example_impl/ntt_impl.cpp:101-108 forward, :151-157 inverse): sums in
+// the element type T, products in the double width T_DP.  A negative input
+// (and, for the inverse, one at or above q) therefore propagates negative
+// representatives in (-q, q): congruent to what the passes compute, but not
+// equal.  For every column flagged by the first pass this kernel replays the
+// reference network step for step on the column's original input: one
+// workgroup per column, the values in a global scratch row, one barrier per
+// stage.  Sums wrap in 64 bits like the reference's int64 arithmetic; int32
+// values are cut to 32 bits where the reference stores them in T.  The INTT's
+// m^-1 scaling is exact (the reference multiplies in T, which overflows for
+// q > ~2^31.5 in int64).  The kernel clears each flag it has served.
+struct RefArgs {
+    ColAddr src;            // a
+    ColAddr old;            // the result's previous contents (rows an inverse scatter does not hit)
+    ColAddr dst;            // result
+    int64_t* scratch;       // [gridDim.x][m]
+    uint32_t* colflag;      // [ncols]
+    const int32_t* rowmap;  // the plan's row map (meaning per perm_kind)
+    int32_t perm_kind;
+    int32_t inverse;
+    int32_t is32;
+    int32_t logm;
+    int64_t m, ncols;
+    int64_t cdim[3];
+    int32_t limb_axis;
+    const LimbInfo* limbs;
+    const uint64_t* tw;      // reduced table pairs, equal to the given table unless raw_tw is set
+    const int64_t* raw_tw;   // [k][m] or null
+    const int64_t* raw_minv; // [k] or null
+};
+
+__device__ __forceinline__ int64_t ref_t(int64_t x, int is32) { return is32 ? (int64_t)(int32_t)x : x; }
+
+// static_cast<T>(T_DP(x) * T_DP(y) % T_DP(q)) for x, y of type T
+__device__ __forceinline__ int64_t ref_mulmod(const ModInfo& mi, int is32, int64_t x, int64_t y) {
+    if (is32) return cpp_rem(mi, x * y);  // |x|, |y| < 2^31: exact in int64; |result| < q < 2^31
+    const bool neg = (x < 0) != (y < 0);
+    const uint64_t ux = uabs64(x), uy = uabs64(y);
+    const uint64_t r = reduce128(mi, reduce64(mi, mulhi64(ux, uy)), ux * uy);
+    return apply_sign(neg && r != 0, r);
+}
+
+__global__ __launch_bounds__(256) void k_ntt_ref(RefArgs A) {
+    const int64_t m = A.m, half = m >> 1;
+    int64_t* x = A.scratch + (int64_t)blockIdx.x * m;
+    for (int64_t col = blockIdx.x; col < A.ncols; col += gridDim.x) {
+        if (!A.colflag[col]) continue;  // the same value for every thread of the workgroup
+        int64_t c = col, coord[3];
+        coord[0] = c % A.cdim[0];
+        c /= A.cdim[0];
+        coord[1] = c % A.cdim[1];
+        coord[2] = c / A.cdim[1];
+        const int limb = (int)coord[A.limb_axis];
+        const LimbInfo& L = A.limbs[limb];
+        const ModInfo mi = limb_modinfo(L);
+        const uint64_t q = L.q;
+        const int64_t so = col_offset(A.src, coord), dof = col_offset(A.dst, coord);
+        const int64_t oo = A.old.ptr ? col_offset(A.old, coord) : 0;
+        // the network's input: a, scattered by perm for the inverse
+        // (x[perm[u]] = a[u]; rows hit by no entry keep the result's contents)
+        for (int64_t j = threadIdx.x; j < m; j += blockDim.x) {
+            int64_t v;
+            if (A.inverse && A.perm_kind == PERM_MAP) {
+                v = load_any(A.src, so + (int64_t)A.rowmap[j] * A.src.row_stride);
+            } else if (A.inverse && A.perm_kind == PERM_SCATTER_INV) {
+                const int32_t r = A.rowmap[j];
+                v = r >= 0 ? load_any(A.src, so + (int64_t)r * A.src.row_stride)
+                           : load_any(A.old, oo + j * A.old.row_stride);
+            } else {
+                v = load_any(A.src, so + j * A.src.row_stride);
+            }
+            x[j] = v;
+        }
+        __syncthreads();
+        const int64_t* twr = A.raw_tw ? A.raw_tw + (int64_t)limb * m : nullptr;
+        const uint64_t* twp = A.tw + 2 * (int64_t)limb * m;
+        if (!A.inverse) {
+            // stage s = 2^lg: butterflies (jx, jx + step), step = m / 2s, twiddle tw[t, s + u]
+            for (int lg = 0; lg < A.logm; ++lg) {
+                const int sb = A.logm - 1 - lg;
+                const int64_t stage = int64_t(1) << lg, step = int64_t(1) << sb;
+                for (int64_t b = threadIdx.x; b < half; b += blockDim.x) {
+                    const int64_t u = b >> sb, jx = (u << (sb + 1)) + (b & (step - 1));
+                    const int64_t s = twr ? twr[stage + u] : (int64_t)twp[2 * (stage + u)];
+                    const int64_t uv = x[jx];
+                    const int64_t vm = ref_mulmod(mi, A.is32, x[jx + step], s);
+                    x[jx] = ref_t(cpp_rem(mi, (int64_t)((uint64_t)uv + (uint64_t)vm)), A.is32);
+                    x[jx + step] = ref_t(cpp_rem(mi, (int64_t)((uint64_t)uv + q - (uint64_t)vm)), A.is32);
+                }
+                __syncthreads();
+            }
+            // out[u] = x[perm[u]]
+            for (int64_t j = threadIdx.x; j < m; j += blockDim.x) {
+                int64_t row = j, v = x[j];
+                if (A.perm_kind == PERM_MAP) row = A.rowmap[j];
+                else if (A.perm_kind == PERM_GATHER_FWD) v = x[A.rowmap[j]];
+                store_any(A.dst, dof + row * A.dst.row_stride, (uint64_t)v);
+            }
+        } else {
+            // half = m/2 .. 1 with t_stride = 1 .. m/2, twiddle itw[t, half + group]
+            for (int lg = 0; lg < A.logm; ++lg) {
+                const int64_t ts = int64_t(1) << lg, hf = half >> lg;
+                for (int64_t tid = threadIdx.x; tid < half; tid += blockDim.x) {
+                    const int64_t group = tid >> lg;
+                    const int64_t iu = (group << (lg + 1)) + (tid & (ts - 1)), iv = iu + ts;
+                    const int64_t s = twr ? twr[hf + group] : (int64_t)twp[2 * (hf + group)];
+                    const int64_t uv = x[iu], vv = x[iv];
+                    x[iu] = ref_t(cpp_rem(mi, (int64_t)((uint64_t)uv + (uint64_t)vv)), A.is32);
+                    x[iv] = ref_mulmod(mi, A.is32, ref_t((int64_t)((uint64_t)uv + q - (uint64_t)vv), A.is32), s);
+                }
+                __syncthreads();
+            }
+            const int64_t mv = A.raw_minv ? A.raw_minv[limb] : (int64_t)L.minv;
+            for (int64_t j = threadIdx.x; j < m; j += blockDim.x)
+                store_any(A.dst, dof + j * A.dst.row_stride, (uint64_t)ref_mulmod(mi, A.is32, x[j], mv));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) A.colflag[col] = 0u;
+    }
+}
+
+// raw copies of the tables for ref_all plans: out[t][i] = tw[t, i], minv_out[t] = m_inv[t]
+__global__ void k_plan_raw(PlanSrc tw, PlanSrc minv, int has_minv, int64_t k, int64_t m, int64_t* out,
+                           int64_t* minv_out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k * m; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = e / m, i = e - t * m;
+        out[e] = ld_elem(tw.ptr, t * tw.s0 + i * tw.s1, tw.elem32);
+        if (has_minv && i == 0) minv_out[t] = ld_elem(minv.ptr, t * minv.s0, minv.elem32);
     }
 }
 

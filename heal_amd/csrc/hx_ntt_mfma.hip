@@ -36,8 +36,7 @@ struct hx_ntt_mfma_plan {
     uint4* w1;      // same for W1
     uint32_t* tmat; // [k][256 u2][256 i1]
     uint64_t* red;  // [k][10]: q, mu, R_0..R_6, pad
-    mutable void* ws;
-    mutable size_t ws_bytes;
+    hx::Workspaces* wss;  // the step-1 intermediate, per stream (see hx::Workspaces)
 };
 
 namespace hx {
@@ -230,10 +229,10 @@ hx_ntt_mfma_plan* ntt_mfma_plan_create(int64_t k, const void* w2, const void* w1
     std::memset(plan, 0, sizeof *plan);
     plan->k = k;
     check_hip(hipGetDevice(&plan->device), "hipGetDevice");
-    hipError_t e = hipMalloc(&plan->mem, 2 * wb + tb + rb);
+    hipError_t e = hipMallocAsync(&plan->mem, 2 * wb + tb + rb, st);
     if (e != hipSuccess) {
         delete plan;
-        check_hip(e, "hipMalloc(mfma ntt plan)");
+        check_hip(e, "hipMallocAsync(mfma ntt plan)");
     }
     char* m = (char*)plan->mem;
     plan->w2 = (uint4*)m;
@@ -246,8 +245,11 @@ hx_ntt_mfma_plan* ntt_mfma_plan_create(int64_t k, const void* w2, const void* w1
         check_hip(hipMemcpyAsync(plan->tmat, tmat, tb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
         check_hip(hipMemcpyAsync(plan->red, red, rb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
         check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        plan->wss = new Workspaces;
+        plan->wss->device = plan->device;
     } catch (...) {
-        (void)hipFree(plan->mem);
+        (void)hipFreeAsync(plan->mem, st);
+        (void)hipStreamSynchronize(st);
         delete plan;
         throw;
     }
@@ -265,44 +267,42 @@ void ntt_mfma_exec(const hx_ntt_mfma_plan* plan, const hx_view* a, const hx_view
     const int64_t P = a->dims[0], K = plan->k;
     if (P == 0) return;
     const size_t bytes = (size_t)P * K * kRows * kRows * 4;
-    if (plan->ws_bytes < bytes) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
-        if (cs != hipStreamCaptureStatusNone)
-            raise(HX_ERR_RUNTIME, "the plan's workspace must be sized by one execution before graph capture");
-        if (plan->ws) check_hip(hipFreeAsync(plan->ws, st), "hipFreeAsync");
-        plan->ws = nullptr;
-        plan->ws_bytes = 0;
-        check_hip(hipMallocAsync(&plan->ws, bytes, st), "hipMallocAsync(mfma ntt workspace)");
-        plan->ws_bytes = bytes;
-    }
+    if (views_overlap(a, out) && !(a->data == out->data)) raise(HX_ERR_INVALID_ARGUMENT, "a and result overlap");
+    std::lock_guard<std::mutex> lk(plan->wss->mu);
+    const Workspaces::Ws w = plan->wss->acquire(st, 0, bytes);
     Ntt4Args A;
     std::memset(&A, 0, sizeof A);
     A.a = (const int64_t*)a->data;
     A.a_sp = a->strides[0], A.a_st = a->strides[3], A.a_si = a->strides[1];
     A.out = (int64_t*)out->data;
     A.o_sp = out->strides[0], A.o_st = out->strides[3], A.o_si = out->strides[1];
-    A.mid = (uint32_t*)plan->ws;
+    A.mid = (uint32_t*)w.data;
     A.tmat = plan->tmat;
     A.red = plan->red;
     A.P = P;
     A.K = K;
     const int64_t blocks = P * K * kColBlocks * kRowHalves;
     if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "too many columns for one launch");
-    if (views_overlap(a, out) && !(a->data == out->data)) raise(HX_ERR_INVALID_ARGUMENT, "a and result overlap");
     A.w = plan->w2;
     hipLaunchKernelGGL(k_ntt4<1>, dim3((unsigned)blocks), dim3(kThreads4), 0, st, A);
     check_hip(hipGetLastError(), "mfma ntt step 1 launch");
     A.w = plan->w1;
     hipLaunchKernelGGL(k_ntt4<2>, dim3((unsigned)blocks), dim3(kThreads4), 0, st, A);
     check_hip(hipGetLastError(), "mfma ntt step 2 launch");
+    plan->wss->note(st);
 }
 
+// freed after the work already queued with the plan (stream-ordered, no host wait)
 void ntt_mfma_plan_destroy(hx_ntt_mfma_plan* plan) {
     if (!plan) return;
-    (void)hipDeviceSynchronize();
-    (void)hipFree(plan->mem);
-    if (plan->ws) (void)hipFree(plan->ws);
+    std::vector<hipEvent_t> events;
+    std::vector<void*> ptrs{plan->mem};
+    {
+        std::lock_guard<std::mutex> lk(plan->wss->mu);
+        plan->wss->release(events, ptrs);
+    }
+    release_after(plan->device, events, ptrs);
+    delete plan->wss;
     delete plan;
 }
 

@@ -3,6 +3,7 @@
 // Replaces the storage layer of the reference's DeviceTensor
 // (example_impl/device_memory_impl.cpp:14-202), which keeps tensors in host
 // malloc memory, with HBM from a stream-ordered caching pool.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -163,10 +164,12 @@ DevStatus* device_status() {
     std::lock_guard<std::mutex> lk(g_status_mu);
     if ((int)g_status.size() <= dev) g_status.resize(dev + 1, nullptr);
     if (!g_status[dev]) {
+        // once per device: the block is zeroed before any kernel can see it
         DevStatus* p = nullptr;
         check_hip(hipMalloc((void**)&p, sizeof(DevStatus)), "hipMalloc(status)");
-        check_hip(hipMemset(p, 0, sizeof(DevStatus)), "hipMemset(status)");
-        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check_hip(hipMemcpy(p, std::vector<char>(sizeof(DevStatus), 0).data(), sizeof(DevStatus),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy(status)");
         g_status[dev] = p;
         // keep freed pool memory cached in the process instead of returning it
         hipMemPool_t pool;
@@ -225,6 +228,177 @@ void fill_zero(void* ptr, size_t bytes, hipStream_t st) {
     check_hip(hipGetLastError(), "fill launch");
 }
 
+__global__ void k_status_take(unsigned long long* st, unsigned long long* out, int n) {
+    const int i = threadIdx.x;
+    if (i < n) out[i] = atomicExch(&st[i], 0ull);
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_u32(uint32_t* ptr, int64_t n, uint32_t value, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, ptr, n, value);
+    check_hip(hipGetLastError(), "fill launch");
+}
+
+// --------------------------------------------------- side and reclaim streams
+// Two library streams per device (default priority, non-blocking): the side
+// stream runs host-synchronising work while a caller's stream is being
+// captured, the reclaim stream orders frees after the work that used the
+// memory.  They live until the process ends.
+struct DevStreams {
+    hipStream_t side = nullptr;
+    hipStream_t reclaim = nullptr;
+};
+static std::mutex g_streams_mu;
+static std::vector<DevStreams> g_streams;
+
+static DevStreams& dev_streams(int dev) {
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    if ((int)g_streams.size() <= dev) g_streams.resize(dev + 1);
+    DevStreams& s = g_streams[dev];
+    if (!s.side) {
+        int cur = 0;
+        check_hip(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != dev) check_hip(hipSetDevice(dev), "hipSetDevice");
+        hipError_t e1 = hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking);
+        hipError_t e2 = hipStreamCreateWithFlags(&s.reclaim, hipStreamNonBlocking);
+        if (cur != dev) (void)hipSetDevice(cur);
+        check_hip(e1, "hipStreamCreateWithFlags");
+        check_hip(e2, "hipStreamCreateWithFlags");
+    }
+    return s;
+}
+
+bool stream_capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+    return cs != hipStreamCaptureStatusNone;
+}
+
+SyncScope::SyncScope(hipStream_t st) : st_(st), escaped_(false), mode_(0) {
+    if (!stream_capturing(st)) return;
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    hipStreamCaptureMode m = hipStreamCaptureModeRelaxed;
+    check_hip(hipThreadExchangeStreamCaptureMode(&m), "hipThreadExchangeStreamCaptureMode");
+    mode_ = (int)m;
+    escaped_ = true;
+    st_ = dev_streams(dev).side;
+}
+
+SyncScope::~SyncScope() {
+    if (!escaped_) return;
+    hipStreamCaptureMode m = (hipStreamCaptureMode)mode_;
+    (void)hipThreadExchangeStreamCaptureMode(&m);
+}
+
+void SyncScope::sync() { check_hip(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
+
+void release_after(int device, std::vector<hipEvent_t>& events, const std::vector<void*>& ptrs) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = device;
+    if (cur != device) (void)hipSetDevice(device);
+    hipStream_t rs = nullptr;
+    try {
+        rs = dev_streams(device).reclaim;
+    } catch (...) {
+        rs = nullptr;
+    }
+    for (hipEvent_t ev : events) {
+        if (rs) (void)hipStreamWaitEvent(rs, ev, 0);
+        (void)hipEventDestroy(ev);
+    }
+    events.clear();
+    for (void* p : ptrs)
+        if (p) {
+            if (rs) (void)hipFreeAsync(p, rs);
+            else (void)hipFree(p);
+        }
+    if (cur != device) (void)hipSetDevice(cur);
+}
+
+// ----------------------------------------------------------------- workspaces
+// A workspace allocated for a capture is made outside the graph: on the side
+// stream with the capture mode relaxed, flags zeroed there, then waited for.
+static Workspaces::Ws alloc_ws(hipStream_t st, bool capturing, size_t flag_bytes, size_t data_bytes) {
+    Workspaces::Ws w;
+    SyncScope sc(st);
+    hipStream_t s = sc.stream();
+    const size_t fb = (flag_bytes + 255) / 256 * 256;
+    if (fb + data_bytes == 0) return w;
+    void* p = nullptr;
+    check_hip(hipMallocAsync(&p, fb + data_bytes, s), "hipMallocAsync(workspace)");
+    if (fb) fill_zero(p, fb, s);
+    if (capturing) sc.sync();
+    w.flags = fb ? p : nullptr;
+    w.flag_bytes = fb;
+    w.data = data_bytes ? (char*)p + fb : nullptr;
+    w.data_bytes = data_bytes;
+    return w;
+}
+
+static void* ws_base(const Workspaces::Ws& w) { return w.flags ? w.flags : w.data; }
+
+Workspaces::Ws Workspaces::acquire(hipStream_t st, size_t flag_bytes, size_t data_bytes) {
+    if (stream_capturing(st)) {
+        // the graph keeps this workspace; it is released with the plan
+        Ws w = alloc_ws(st, true, flag_bytes, data_bytes);
+        captured.push_back(w);
+        return w;
+    }
+    for (Eager& e : eager) {
+        if (e.s != st) continue;
+        if (e.ws.flag_bytes >= flag_bytes && e.ws.data_bytes >= data_bytes) return e.ws;
+        // grown in this stream's order: earlier work on st still reads the old one
+        const size_t fb = std::max(flag_bytes, e.ws.flag_bytes), db = std::max(data_bytes, e.ws.data_bytes);
+        if (void* b = ws_base(e.ws)) check_hip(hipFreeAsync(b, st), "hipFreeAsync(workspace)");
+        e.ws = Ws();
+        e.ws = alloc_ws(st, false, fb, db);
+        return e.ws;
+    }
+    Eager e;
+    e.s = st;
+    e.ev = nullptr;
+    check_hip(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming), "hipEventCreate");
+    try {
+        e.ws = alloc_ws(st, false, flag_bytes, data_bytes);
+    } catch (...) {
+        (void)hipEventDestroy(e.ev);
+        throw;
+    }
+    eager.push_back(e);
+    return e.ws;
+}
+
+void Workspaces::note(hipStream_t st) {
+    if (stream_capturing(st)) return;
+    for (Eager& e : eager)
+        if (e.s == st) {
+            check_hip(hipEventRecord(e.ev, st), "hipEventRecord");
+            return;
+        }
+    // a stream that needed no workspace still uses the plan's tables
+    Eager e;
+    e.s = st;
+    check_hip(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming), "hipEventCreate");
+    eager.push_back(e);
+    check_hip(hipEventRecord(e.ev, st), "hipEventRecord");
+}
+
+void Workspaces::release(std::vector<hipEvent_t>& events, std::vector<void*>& ptrs) {
+    for (Eager& e : eager) {
+        events.push_back(e.ev);
+        ptrs.push_back(ws_base(e.ws));
+    }
+    for (Ws& w : captured) ptrs.push_back(ws_base(w));
+    eager.clear();
+    captured.clear();
+}
+
 }  // namespace hx
 
 using namespace hx;
@@ -237,12 +411,21 @@ const char* hx_last_error(void) { return t_err.c_str(); }
 
 int hx_status_poll(hx_stream stream, hx_device_status* out) {
     HX_API_BEGIN
+    // in the stream's order: counters are taken (read and cleared atomically)
+    // after this stream's earlier work; counts of work still running on other
+    // streams stay for the next poll
     DevStatus* st = device_status();
-    check_hip(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+    hipStream_t s = as_stream(stream);
+    const int n = (int)(sizeof(DevStatus) / sizeof(unsigned long long));
+    void* tmp = nullptr;
+    check_hip(hipMallocAsync(&tmp, sizeof(DevStatus), s), "hipMallocAsync(status)");
     DevStatus h;
-    check_hip(hipMemcpy(&h, st, sizeof h, hipMemcpyDeviceToHost), "hipMemcpy(status)");
-    check_hip(hipMemset(st, 0, sizeof h), "hipMemset(status)");
-    check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hipLaunchKernelGGL(k_status_take, dim3(1), dim3(64), 0, s, (unsigned long long*)st, (unsigned long long*)tmp, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, tmp, sizeof h, hipMemcpyDeviceToHost, s);
+    (void)hipFreeAsync(tmp, s);
+    check_hip(e, "status take");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (out) {
         out->gdecomp_overflow = h.gdecomp_overflow;
         out->index_out_of_range = h.index_out_of_range;
@@ -298,6 +481,13 @@ int hx_memcpy_d2d(void* dst, const void* src, size_t bytes, hx_stream stream) {
 int hx_stream_sync(hx_stream stream) {
     HX_API_BEGIN
     check_hip(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+    HX_API_END
+}
+
+int hx_stream_capturing(hx_stream stream, int* capturing) {
+    HX_API_BEGIN
+    if (!capturing) raise(HX_ERR_INVALID_ARGUMENT, "capturing pointer must not be null.");
+    *capturing = stream_capturing(as_stream(stream)) ? 1 : 0;
     HX_API_END
 }
 

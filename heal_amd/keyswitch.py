@@ -48,6 +48,7 @@ class KeySwitchBatch:
         self.inv = capi.NttPlan(self.prm.p, self.prm.perm, self.prm.inv_twiddles, self.prm.m_inv, inverse=True)
         self.graph = None
         self.fused = None
+        self.fused_ok = True  # False: the fused kernels refused these inputs; run_fused runs the chain
         self.key = None
         self.perms = perms
 
@@ -70,17 +71,36 @@ class KeySwitchBatch:
             self.fused.close()
             self.fused = None
             self.key = None
+        self.fused_ok = True
 
     # -- fused form: three kernels, key in device layout ----------------------
     def prepare_fused(self):
-        if self.fused is None:
-            self.fused = capi.KeySwitchPlan(self.fwd, self.inv, self.perms, self.dnum, self.w)
-            self.key = self.fused.prepare_key(self.evk)
+        """The fused plan and device key, or None when the fused kernels refuse
+        the inputs: keys with negative words (the chain then keeps the signed
+        values of the reference's modmul through its INTT), tables outside
+        [0, q) or digits too wide for the moduli.  Those run the chain."""
+        if self.fused is None and self.fused_ok:
+            plan = None
+            try:
+                plan = capi.KeySwitchPlan(self.fwd, self.inv, self.perms, self.dnum, self.w)
+                self.key = plan.prepare_key(self.evk)
+            except ValueError as e:
+                if "six-call chain" not in str(e):
+                    raise
+                if plan is not None:
+                    plan.close()
+                self.fused_ok = False
+                return None
+            self.fused = plan
         return self.fused
 
     def run_fused(self, out=None):
-        self.prepare_fused()
         out = self.out if out is None else out
+        if self.prepare_fused() is None:
+            res = self.run()
+            if out is not res:
+                out.copy_(res)
+            return out
         self.fused(self.x, self.key, out)
         return out
 

@@ -66,6 +66,7 @@ class _Segment:
         self.ops = ops  # entry indices of DEVICE_OPs
         self.capturable = True
         self.graph = None
+        self.pins = None  # the backend's record of the plans the graph uses
         self.inputs = {}  # name -> (buffer id, offset, dims, strides) at capture
         self.out_arg = {}  # entry index -> position of the returned object among the op's args
         self.tables = set()  # names read in a table role
@@ -190,6 +191,7 @@ class TranscriptProgram:
         g = torch.cuda.CUDAGraph()
         old = m.get_stream()
         m.set_stream(stream.cuda_stream)
+        m.capture_begin()
         try:
             with torch.cuda.graph(g, stream=stream):
                 for j in seg.ops:
@@ -207,8 +209,9 @@ class TranscriptProgram:
                             pos = [k for k, a in enumerate(args) if a is out]
                             seg.out_arg[j] = pos[0] if pos else None
         finally:
+            pins = m.capture_end()
             m.set_stream(old)
-        seg.graph = g
+        seg.graph, seg.pins = g, pins
         g.replay()
 
     def _replay(self, seg, refs):
@@ -231,6 +234,13 @@ class TranscriptProgram:
                     raise RuntimeError(f"segment {seg.name!r}: cannot rebind the result of {op.name}")
                 refs[op.out.value.inf_name] = _arg_value(refs, op.args[pos])
         seg.graph.replay()
+
+    @staticmethod
+    def _drop_graph(seg):
+        """Releases a segment's graph, then the plans it kept (after it ran)."""
+        torch.cuda.synchronize()
+        seg.graph = None
+        seg.pins = None
 
     def _graph_usable(self, seg, refs, changed):
         """The graph still addresses every input: same buffer, offset and layout,
@@ -269,7 +279,7 @@ class TranscriptProgram:
                 if self.graphs and seg.capturable:
                     state = self._input_state(seg, refs)
                     if seg.graph is not None and (seg.tables & changed):
-                        seg.graph = None  # the graph holds the old tables' prepared data
+                        self._drop_graph(seg)  # the graph holds the old tables' prepared data
                     if seg.graph is not None and state == seg.inputs:
                         mode = "replay"
                     elif getattr(seg, "eager_inputs", None) == state and not (seg.tables & changed):

@@ -18,10 +18,17 @@
  *  - A view's `data` points at element [0,...,0]; strides may be 0 (broadcast).
  *  - Element types: HX_INT32, HX_INT64 for compute ops; HX_FLOAT64 only for the
  *    memory/view ops, as in the reference.
- *  - Results are exact integers.  Inputs of the modular ops follow C++ integer
- *    semantics of the reference (truncating remainder, 128-bit/64-bit
- *    intermediates).  NTT/INTT inputs must be non-negative; outputs are the
- *    canonical residues in [0, q).
+ *  - Results are exact integers and equal the reference's wherever the
+ *    reference is defined: C++ integer semantics (truncating remainder,
+ *    128-bit/64-bit intermediates), including the NTT/INTT of inputs outside
+ *    [0, q), whose signed representatives the reference's network keeps.
+ *    Where the reference overflows (its INTT's m^-1 product in T) the result
+ *    is the exact one.
+ *  - Plans (NTT, key-switch, matrix-core NTT) may be used from several threads
+ *    and streams at once: each stream gets its own workspace.  Destroying a
+ *    plan never waits on the host; its memory is released after the work
+ *    already queued with it.  A graph captured with a plan must be destroyed
+ *    (and have finished) before the plan.
  */
 #ifndef HEAL_AMD_H
 #define HEAL_AMD_H
@@ -65,8 +72,10 @@ typedef struct hx_device_status {
 /* ------------------------------------------------------------------ library */
 int         hx_abi_version(void);
 const char* hx_last_error(void);
-/* Synchronises `stream`, copies the current device's status counters into *out
- * (may be NULL) and clears them. */
+/* Takes the current device's status counters in `stream`'s order (atomically
+ * read and cleared after the stream's earlier work; counts of work still
+ * running on other streams remain for a later poll), copies them into *out
+ * (may be NULL) and synchronises `stream` only. */
 int hx_status_poll(hx_stream stream, hx_device_status* out);
 
 /* ---------------------------------------------------- memory (L0 of the API)
@@ -81,6 +90,9 @@ int hx_memcpy_h2d(void* dst, const void* src, size_t bytes, hx_stream stream); /
 int hx_memcpy_d2h(void* dst, const void* src, size_t bytes, hx_stream stream); /* returns when dst is filled */
 int hx_memcpy_d2d(void* dst, const void* src, size_t bytes, hx_stream stream);
 int hx_stream_sync(hx_stream stream);
+/* *capturing = 1 while work enqueued on `stream` is being captured into a graph.
+ * No counterpart in the reference (its API has no streams). */
+int hx_stream_capturing(hx_stream stream, int* capturing);
 
 /* ------------------------------------------------------------ views (L1)
  * make_contiguous (reference include/contiguous.h:19-20, example_impl/contiguous_impl.cpp:9-52):
@@ -140,7 +152,12 @@ int hx_g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, 
  * (reference include/ntt.h:31-52, example_impl/ntt_impl.cpp:69-170).
  * log2p_list and mu_list may be NULL and are not read (as in the reference).
  * These build a temporary plan (see hx_ntt_plan_*) and synchronise the stream
- * once to classify `perm`. */
+ * once to classify `perm`.
+ * Columns whose reference result is not the canonical residue (negative
+ * inputs, INTT inputs outside [0, q), forward inputs above 2^63 - 1 - q) are
+ * recomputed by a kernel that replays the reference's signed network; tables
+ * with entries outside [0, q), and moduli above 2^62 (int64) or, for the
+ * int32 INTT, 2^30, send every column there. */
 int hx_ntt(const hx_view* a, const hx_view* p, const hx_view* perm, const hx_view* twiddles,
            const hx_view* log2p_list, const hx_view* mu_list, const hx_view* result, hx_stream stream);
 int hx_intt(const hx_view* a, const hx_view* p, const hx_view* perm, const hx_view* inv_twiddles,
@@ -166,6 +183,13 @@ int hx_ntt_plan_create(hx_ntt_plan** out, int inverse, const hx_view* p, const h
                        const hx_view* twiddles, const hx_view* m_inv, hx_stream stream);
 int hx_ntt_plan_exec(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream);
 int hx_ntt_plan_destroy(hx_ntt_plan* plan);
+/* hx_ntt_plan_exec with a caller-provided workspace (256-byte aligned, at
+ * least the size hx_ntt_plan_workspace_bytes gives for the same views; its
+ * contents on entry do not matter and it must not be shared by executions in
+ * flight at the same time).  No counterpart in the reference. */
+int hx_ntt_plan_workspace_bytes(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, size_t* bytes);
+int hx_ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, void* workspace,
+                        size_t workspace_bytes, hx_stream stream);
 /* permute without the host synchronisation: perms must have been validated
  * (e.g. by an earlier hx_permute); out-of-range entries still never read
  * outside `a`, they write 0 and count in index_out_of_range. */
@@ -193,6 +217,12 @@ int hx_perms_rows_same(const hx_view* perms, int* same, hx_stream stream);
  * hx_keyswitch_prepare_key into a caller-allocated buffer of
  * hx_keyswitch_key_bytes(plan, B) bytes; exec then runs three kernels and never
  * synchronises.  g_decomposition overflow is counted as by hx_g_decomposition.
+ * The fused result equals the chain's (and the reference's) when every key word
+ * is >= 0: prepare_key synchronises once and rejects a key with negative words
+ * (HX_ERR_INVALID_ARGUMENT), whose chain result carries the reference's signed
+ * INTT representatives; plan creation rejects tables outside [0, q), moduli
+ * above 2^62 and digit sizes with 2^w - 1 + q > 2^63 - 1.  Such inputs belong to
+ * the six-call chain.
  * Plan creation times a few launch shapes on scratch data (the split of log2 m
  * between the high and the low window, chosen once per (m, dnum) in a process,
  * and the limbs per inner workgroup, per (m, k, dnum)); every choice gives the

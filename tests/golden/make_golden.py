@@ -11,6 +11,10 @@ butterfly network, by the defining sum out[u] = sum_i a_i psi^((2u+1) i) mod q
 (with the bit-reversed psi-power twiddle tables of heal_amd.params), at m = 16
 and 64 for 30-bit and 60-bit primes.
 
+ntt_signed_kat.json holds length-2 NTT/INTT vectors worked by hand from the
+reference's loops on inputs outside [0, q), where its truncating % keeps
+negative representatives (see ntt_signed_cases).
+
 Run:  python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -292,9 +296,51 @@ def ntt_closed_form_cases():
     return out
 
 
+def ntt_signed_cases():
+    """Length-2 NTT/INTT vectors worked by hand from the reference's loops, with
+    C++'s truncating % (example_impl/ntt_impl.cpp:101-108 forward, :151-157
+    inverse, :165 scaling).  q = 17, twiddles [1, 4] (psi = 4 is a primitive
+    4th root), inverse twiddles [1, 13] (13 = 4^-1), m_inv = 9 (= 2^-1).
+    Forward:  x0 = (a0 + v) % q, x1 = (a0 + q - v) % q with v = (a1 * 4) % q,
+              then out[u] = x[perm[u]].
+    Inverse:  x[perm[u]] = a[u]; x0 = (x0 + x1) % q, x1 = ((x0 + q - x1) * 13) % q,
+              then each times 9, % q.
+    Worked examples (the sign of every remainder is the sign of its dividend):
+      fwd [-1, 0]:   v = 0;  (-1) % 17 = -1;  16 % 17 = 16                  -> [-1, 16]
+      fwd [3, -2]:   v = -8; (-5) % 17 = -5;  28 % 17 = 11                  -> [-5, 11]
+      fwd [3, -2], perm [1, 0]                                              -> [11, -5]
+      fwd [-20, -3]: v = -12; (-32) % 17 = -15; 9 % 17 = 9                  -> [-15, 9]
+      inv [-5, 3]:   -2; 9 * 13 = 117 -> 15; -18 % 17 = -1; 135 % 17 = 16  -> [-1, 16]
+      inv [2, 30]:   32 -> 15; -11 * 13 = -143 -> -7; 135 -> 16; -63 -> -12 -> [16, -12]
+      inv [2, 30], perm [1, 0]: x = [30, 2]: 15; 45 * 13 = 585 -> 7; 16; 63 -> 12 -> [16, 12]
+      inv [-20, -3]: -23 -> -6; 0; -54 -> -3; 0                             -> [-3, 0]
+    Every case is stored for int32 and int64 as an [l=1, m=2, r=1, k=1] tensor."""
+    base = {"p": [17], "tw": [[1, 4]], "itw": [[1, 13]], "m_inv": [9]}
+    rows = [
+        ("fwd", [-1, 0], [0, 1], [-1, 16]),
+        ("fwd", [3, -2], [0, 1], [-5, 11]),
+        ("fwd", [3, -2], [1, 0], [11, -5]),
+        ("fwd", [-20, -3], [0, 1], [-15, 9]),
+        ("inv", [-5, 3], [0, 1], [-1, 16]),
+        ("inv", [2, 30], [0, 1], [16, -12]),
+        ("inv", [2, 30], [1, 0], [16, 12]),
+        ("inv", [-20, -3], [0, 1], [-3, 0]),
+    ]
+    cases = []
+    for dtype in ("int32", "int64"):
+        for i, (kind, a, perm, want) in enumerate(rows):
+            c = dict(base)
+            c.update({"name": f"signed_{kind}_{i}_{dtype}", "kind": kind, "dtype": dtype, "a": a, "perm": perm,
+                      "want": want})
+            cases.append(c)
+    return cases
+
+
 if __name__ == "__main__":
     with open(os.path.join(HERE, "reference_kat.json"), "w") as f:
         json.dump(reference_cases(), f, indent=0)
     with open(os.path.join(HERE, "ntt_closed_form.json"), "w") as f:
         json.dump(ntt_closed_form_cases(), f, indent=0)
+    with open(os.path.join(HERE, "ntt_signed_kat.json"), "w") as f:
+        json.dump(ntt_signed_cases(), f, indent=0)
     print("wrote", os.listdir(HERE))

@@ -73,9 +73,11 @@ def test_fused_keyswitch_vs_oracle(gpu, n, limbs, dnum, w, batch):
     ks.close()
 
 
-def test_fused_keyswitch_negative_key_matches_chain(gpu):
-    """Negative key entries give signed products and digit sums (C++ truncation);
-    the INTT then reduces its negative inputs mathematically, in both forms."""
+def test_fused_keyswitch_negative_key_matches_oracle(gpu):
+    """Negative key entries give signed products and digit sums (C++
+    truncation), and the chain's INTT keeps the reference's signed
+    representatives: the fused plan refuses such a key, the fused call then
+    runs the chain, and both equal the oracle's chain."""
     from heal_amd.keyswitch import KeySwitchBatch
 
     ks = KeySwitchBatch(n=4096, limbs=3, dnum=3, base_bits=20, batch=2, seed=77)
@@ -83,9 +85,12 @@ def test_fused_keyswitch_negative_key_matches_chain(gpu):
     ks.run()
     torch.cuda.synchronize()
     want = ks.out.clone()
+    for b in range(2):
+        assert torch.equal(want[b].cpu(), oracle_keyswitch(ks, b)), b
     out = torch.zeros_like(want)
     ks.run_fused(out)
     torch.cuda.synchronize()
+    assert not ks.fused_ok
     assert torch.equal(out, want)
     ks.close()
 

@@ -231,3 +231,89 @@ def test_oracle_vs_reference_ntt_intt(m, bits):
         assert OracleBackend().intt(a, prm.p, perm, table, prm.m_inv, i1, mode) is None
         assert RefBackend().intt(a, prm.p, perm, table, prm.m_inv, i2) is None
         assert torch.equal(i1, i2)
+
+
+# ------------------------------------------------------------ signed NTT inputs
+SIGNED = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ntt_signed_kat.json")))
+
+
+def _signed_case_tensors(c):
+    dt = torch.int32 if c["dtype"] == "int32" else torch.int64
+    t = lambda v: torch.tensor(v, dtype=dt)  # noqa: E731
+    return (t(c["a"]).view(1, 2, 1, 1), t(c["p"]), t(c["perm"]), t(c["tw"]), t(c["itw"]), t(c["m_inv"]),
+            t(c["want"]).view(1, 2, 1, 1))
+
+
+def run_signed_case(backend, c, to=lambda x: x):
+    """Runs one hand-worked signed case on a backend; returns the result tensor."""
+    a, p, perm, tw, itw, minv, want = _signed_case_tensors(c)
+    out = to(torch.zeros_like(a))
+    if c["kind"] == "fwd":
+        st = backend.ntt(to(a), to(p), to(perm), to(tw), out)
+    else:
+        st = backend.intt(to(a), to(p), to(perm), to(itw), to(minv), out)
+    assert st is None
+    return out
+
+
+@pytest.mark.parametrize("case", SIGNED, ids=[c["name"] for c in SIGNED])
+def test_oracle_signed_kat(case):
+    """The oracle keeps the reference's negative representatives (hand-worked vectors)."""
+    got = run_signed_case(OracleBackend(), case)
+    assert got.view(-1).tolist() == case["want"]
+
+
+@ref_only
+@pytest.mark.parametrize("case", SIGNED, ids=[c["name"] for c in SIGNED])
+def test_reference_signed_kat(case):
+    """The compiled reference gives the hand-worked vectors too (pins the derivation)."""
+    got = run_signed_case(RefBackend(), case)
+    assert got.view(-1).tolist() == case["want"]
+
+
+@ref_only
+@pytest.mark.parametrize("m,bits,dtype", [(16, 13, torch.int32), (256, 15, torch.int32), (64, 30, torch.int64),
+                                          (1024, 30, torch.int64), (256, 59, torch.int64)])
+def test_oracle_vs_reference_signed_inputs(m, bits, dtype):
+    """Negative inputs (forward) and inputs in [-q, 2q) (inverse), with reduced
+    and with signed, unreduced twiddle tables: the oracle equals the compiled
+    reference (its wrapped scaling, mode 1, above ~31.5-bit primes)."""
+    g = torch.Generator().manual_seed(m * 7 + bits)
+    primes = P.ntt_primes(m, [bits, bits - 1])
+    prm = P.NttParams(m, primes)
+    q = prm.p.view(1, 1, 1, -1)
+    big = 2**29 if dtype == torch.int32 else 2**61
+    a = (torch.randint(-big, big, (2, m, 2, 2), generator=g, dtype=torch.int64)).to(dtype)
+    perm = torch.randperm(m, generator=g).to(dtype)
+    tables = [prm.twiddles.to(dtype)]
+    signed_tw = prm.twiddles - torch.randint(0, 2, prm.twiddles.shape, generator=g) * prm.p.view(-1, 1)
+    tables.append(signed_tw.to(dtype))
+    for table in tables:
+        r1, r2 = torch.zeros_like(a), torch.zeros_like(a)
+        assert OracleBackend().ntt(a, prm.p.to(dtype), perm, table, r1) is None
+        assert RefBackend().ntt(a, prm.p.to(dtype), perm, table, r2) is None
+        assert torch.equal(r1, r2)
+        assert bool((r1 < 0).any())  # the signed representatives are exercised
+        x = (torch.randint(-1, 2, a.shape, generator=g) * q + torch.stack(
+            [torch.randint(0, int(p), (2, m, 2), generator=g) for p in primes], dim=-1)).to(dtype)
+        mode = 1 if bits > 31 else 0
+        i1, i2 = torch.zeros_like(x), torch.zeros_like(x)
+        assert OracleBackend().intt(x, prm.p.to(dtype), perm, table, prm.m_inv.to(dtype), i1, mode) is None
+        assert RefBackend().intt(x, prm.p.to(dtype), perm, table, prm.m_inv.to(dtype), i2) is None
+        assert torch.equal(i1, i2)
+
+
+@pytest.mark.parametrize("m,bits", [(64, 30), (4096, 50)])
+def test_oracle_signed_outputs_are_congruent_representatives(m, bits):
+    """Signed results are representatives in (-q, q) of the same residues as
+    the transform of the reduced input."""
+    g = torch.Generator().manual_seed(m + bits)
+    primes = P.ntt_primes(m, [bits])
+    prm = P.NttParams(m, primes)
+    q = int(primes[0])
+    a = torch.randint(-(2**40), 2**40, (1, m, 1, 1), generator=g, dtype=torch.int64)
+    r_signed, r_red = torch.zeros_like(a), torch.zeros_like(a)
+    assert OracleBackend().ntt(a, prm.p, prm.perm, prm.twiddles, r_signed) is None
+    assert OracleBackend().ntt(a % q, prm.p, prm.perm, prm.twiddles, r_red) is None
+    assert bool((r_signed < 0).any()) and bool((r_signed.abs() < q).all())
+    assert torch.equal(r_signed % q, r_red)
