@@ -87,6 +87,7 @@ _PROTOS = {
     "hx_ntt_plan_destroy": [ctypes.c_void_p],
     "hx_ntt_plan_workspace_bytes": [ctypes.c_void_p, VP, VP, ctypes.POINTER(ctypes.c_size_t)],
     "hx_ntt_plan_exec_ws": [ctypes.c_void_p, VP, VP, ctypes.c_void_p, ctypes.c_size_t, STREAM],
+    "hx_ntt_plan_exec_noalloc": [ctypes.c_void_p, VP, VP, STREAM],
     "hx_keyswitch_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p, VP, U64, U64,
                                  STREAM],
     "hx_keyswitch_key_bytes": [ctypes.c_void_p, I64, ctypes.POINTER(ctypes.c_size_t)],
@@ -368,6 +369,13 @@ class NttPlan:
         _need_cuda(a, out, ws)
         check(lib.hx_ntt_plan_exec_ws(self._h, view(a), view(out), ws.data_ptr(), ws.numel() * ws.element_size(),
                                       stream_of(out)))
+        return out
+
+    def exec_noalloc(self, a, out):
+        """The transform with the library's fixed scratch (no device allocation;
+        calls overlapping on one GPU race)."""
+        _need_cuda(a, out)
+        check(lib.hx_ntt_plan_exec_noalloc(self._h, view(a), view(out), stream_of(out)))
         return out
 
     def close(self):

@@ -23,7 +23,9 @@
 //
 // A plan (hx_ntt_plan) snapshots p, twiddles, perm and m_inv on the device, so
 // executing it reads only the data columns and the packed twiddle table.
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "hx_ntt_kernels.hpp"
@@ -180,7 +182,8 @@ constexpr int64_t kRefBlocks = 1024;
 
 struct ExecShape {
     int64_t ncols;
-    std::vector<int> win;
+    int win[4];  // log2 m <= 30: at most four windows
+    int npass;
     bool need_tmp;
     size_t tmp_bytes;   // T1 (+ T2)
     size_t data_bytes;  // tmp_bytes + fixup scratch
@@ -194,13 +197,14 @@ static ExecShape exec_shape(const hx_ntt_plan* plan, const hx_view* a) {
     const int64_t m = plan->m;
     const int logm = plan->logm;
     if (logm <= 11) {
-        E.win.push_back(logm);
+        E.npass = 1;
+        E.win[0] = logm;
     } else {
-        int npass = (logm + 7) / 8;
-        for (int i = 0; i < npass; ++i) E.win.push_back(logm / npass + (i < logm % npass ? 1 : 0));
+        E.npass = (logm + 7) / 8;
+        for (int i = 0; i < E.npass; ++i) E.win[i] = logm / E.npass + (i < logm % E.npass ? 1 : 0);
     }
     const bool two = plan->perm_kind >= PERM_GATHER_FWD;
-    E.need_tmp = E.win.size() > 1 || two;
+    E.need_tmp = E.npass > 1 || two;
     E.tmp_bytes = E.need_tmp ? (size_t)m * (size_t)E.ncols * 8 * (two ? 2 : 1) : 0;
     E.ref_blocks = std::min<int64_t>(E.ncols, kRefBlocks);
     E.data_bytes = E.tmp_bytes + (E.need_tmp ? 0 : (size_t)E.ref_blocks * (size_t)m * 8);
@@ -261,8 +265,8 @@ static void ntt_plan_run(const hx_ntt_plan* plan, const hx_view* a, const hx_vie
     const int64_t inner_col_stride = A.cstride[0] < 0 ? -A.cstride[0] : A.cstride[0];
     const bool rows_contiguous = (a->strides[1] == 1) && inner_col_stride != 1;
 
-    const std::vector<int>& win = E.win;
-    const int npass = (int)win.size();
+    const int* win = E.win;
+    const int npass = E.npass;
     const int logm = plan->logm;
     void* tmp = E.need_tmp ? data : nullptr;
     void* tmp2 = (E.need_tmp && plan->perm_kind >= PERM_GATHER_FWD) ? (char*)tmp + (size_t)m * ncols * 8 : nullptr;
@@ -417,6 +421,47 @@ void ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view* 
     ntt_plan_run(plan, a, result, E, (uint32_t*)ws, (char*)ws + E.flag_bytes, st);
 }
 
+// ------------------------------------------------------ no-allocation execution
+// The workspace of hx_ntt_plan_exec_noalloc: a device array of fixed size,
+// one copy per GPU, held until the process ends.  Its host-side name is not a
+// device address: kernels and fills get the address hipGetSymbolAddress
+// returns on each device, taken once per device on the first call there.
+static __device__ __align__(256) unsigned char scratch[HX_NTT_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device
+
+// The current device's copy of `scratch` (callers have checked that the
+// current device is the plan's).
+static void* scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) raise(HX_ERR_RUNTIME, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Executes a plan with `scratch` as its workspace: no device memory is
+// allocated.  The flags are cleared by a fill kernel first; the rest of the
+// scratch needs no initial contents.  Every call on a device shares that
+// device's scratch, so calls that overlap in time on one GPU race.
+void ntt_plan_exec_noalloc(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hipStream_t st) {
+    validate_exec(plan, a, result);
+    const ExecShape E = exec_shape(plan, a);
+    if (E.ncols == 0 || plan->m == 0) return;
+    const size_t need = E.flag_bytes + E.data_bytes;
+    if (need > (size_t)HX_NTT_NOALLOC_SCRATCH_BYTES)
+        raise(HX_ERR_INVALID_ARGUMENT, "this transform needs " + std::to_string(need) +
+                                           " bytes of workspace; hx_ntt_plan_exec_noalloc holds " +
+                                           std::to_string((size_t)HX_NTT_NOALLOC_SCRATCH_BYTES));
+    char* ptr = (char*)scratch_ptr(plan->device);
+    fill_zero(ptr, E.flag_bytes, st);
+    ntt_plan_run(plan, a, result, E, (uint32_t*)ptr, ptr + E.flag_bytes, st);
+}
+
 }  // namespace hx
 
 using namespace hx;
@@ -458,6 +503,12 @@ int hx_ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view
                         size_t workspace_bytes, hx_stream stream) {
     HX_API_BEGIN
     ntt_plan_exec_ws(plan, a, result, workspace, workspace_bytes, as_stream(stream));
+    HX_API_END
+}
+
+int hx_ntt_plan_exec_noalloc(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream) {
+    HX_API_BEGIN
+    ntt_plan_exec_noalloc(plan, a, result, as_stream(stream));
     HX_API_END
 }
 

@@ -190,6 +190,18 @@ int hx_ntt_plan_destroy(hx_ntt_plan* plan);
 int hx_ntt_plan_workspace_bytes(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, size_t* bytes);
 int hx_ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, void* workspace,
                         size_t workspace_bytes, hx_stream stream);
+/* hx_ntt_plan_exec that allocates nothing: its workspace is a fixed device
+ * array of HX_NTT_NOALLOC_SCRATCH_BYTES inside the library (one per GPU, held
+ * until the process ends).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Limit: the transform's workspace
+ * (hx_ntt_plan_workspace_bytes: about 8 * m * columns bytes for two or more
+ * passes, twice that for a perm that is not a permutation, plus 4 bytes per
+ * column) must fit, else HX_ERR_INVALID_ARGUMENT.  Every call on a GPU shares
+ * that GPU's array: calls that overlap in time on one GPU (other streams or
+ * threads) race and must be ordered by the caller. */
+#define HX_NTT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hx_ntt_plan_exec_noalloc(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream);
 /* permute without the host synchronisation: perms must have been validated
  * (e.g. by an earlier hx_permute); out-of-range entries still never read
  * outside `a`, they write 0 and count in index_out_of_range. */
