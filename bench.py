@@ -22,10 +22,21 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the forward pass kernel on the bench workload, keyed by
-# (N, L, B): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
-# counter's reading for a 256 MiB copy (profiles/ntt_pass_hbm_traffic.md)
-NTT_PASS_TRAFFIC = {(65536, 24, 16): 1305184556}
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs at 2.4 GHz
+# Profiled constants of the current kernels on the bench workload, keyed by
+# (N, L, B); each entry names the profile it comes from and goes stale when the
+# kernel changes.  HBM bytes per launch: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by the same counter's reading for a 256 MiB copy.
+# VALU instructions per launch: SQ_INSTS_VALU (each issues in one quad-cycle,
+# SQ_ACTIVE_INST_VALU = SQ_INSTS_VALU within 0.2 %).
+PROFILED = {
+    (65536, 24, 16): {
+        "ntt_pass_traffic": (1305184556, "profiles/ntt_pass_hbm_traffic.md"),
+        "ntt_pass_valu": (133886976, "profiles/ntt_pass_sq_counters.md"),
+        "k2_traffic": (2437285478, "profiles/ntt_pass_hbm_traffic.md"),
+        "k2_valu": (259178496, "profiles/ntt_pass_sq_counters.md"),
+    }
+}
 
 
 def parse():
@@ -155,6 +166,18 @@ def main():
 
     t_graph = timed(fused_graph.replay, args.steps, torch)
     t_fused = timed(lambda: batched.run_fused(fused_out), args.steps, torch)
+    # the same K steps again with events between the three kernels (HIP events
+    # on the launch stream), for the per-kernel times of the dominant kernel
+    marks = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    for row in marks:
+        for e in row:
+            e.record()  # creates the underlying event
+    torch.cuda.synchronize()
+    for row in marks:
+        batched.fused.exec_marked(batched.x, batched.key, fused_out, row)
+    torch.cuda.synchronize()
+    k_ms = [sum(row[i].elapsed_time(row[i + 1]) for row in marks) / args.steps for i in range(3)]
+    t_marked = sum(marks[i][0].elapsed_time(marks[i][3]) for i in range(args.steps)) / 1e3
     t_chain_graph = timed(chain_graph.replay, args.steps, torch)
     t_batched = timed(batched.run, args.steps, torch)
     t_plain = timed(plain.run, args.steps * 2, torch) / 2.0
@@ -180,7 +203,7 @@ def main():
     avg_launch = t_ntt / (reps * n_launch)
     achieved = bytes_per_launch / avg_launch / 1e9
     ntt_limb_gbs = ntt_bytes * reps / t_ntt / 1e9
-    traffic = NTT_PASS_TRAFFIC.get((N, L, B))
+    traffic = PROFILED.get((N, L, B), {}).get("ntt_pass_traffic", (None,))[0]
     limbs_inv = B * 2 * L
     t_intt = timed(lambda: batched.inv(batched.acc.view(B * 2, N, 1, L), batched.coef), reps, torch)
     intt_limb_gbs = limbs_inv * 2 * N * 8 * reps / t_intt / 1e9
@@ -209,6 +232,30 @@ def main():
                             "bytes": nbytes, "ms": round(t_op / reps * 1e3, 4)}
     torch.cuda.synchronize()
     assert torch.equal(batched.out, ref_all), "elementwise timing loop changed the chain's output"
+    assert torch.equal(fused_out, ref_all), "the timed fused runs changed their output"
+
+    prof = PROFILED.get((N, L, B), {})
+
+    def profiled(key):
+        v = prof.get(key)
+        return (v[0], v[1]) if v else (None, None)
+
+    # K2: forward low window of the digit columns, key inner product, inverse
+    # low window.  Its bytes as a kernel: the D digit-NTT intermediate columns
+    # and the 2D key columns read, the 2 output columns written, per limb of
+    # every key-switch: (3D + 2) * N * 8 B per limb.
+    k2_bytes = (3 * D + 2) * N * 8 * L * B
+    k2_s = k_ms[1] / 1e3
+    k2_valu, k2_valu_src = profiled("k2_valu")
+    k2_traffic, k2_traffic_src = profiled("k2_traffic")
+    ntt_valu, ntt_valu_src = profiled("ntt_pass_valu")
+
+    def valu_entry(insts, src, seconds):
+        if insts is None:
+            return None
+        issue = insts * 4 / SIMDS / CLOCK_HZ  # quad-cycles on 1024 SIMDs
+        return {"insts_per_launch": insts, "issue_ms_at_one_quad_cycle": round(issue * 1e3, 4),
+                "frac": round(issue / seconds, 3), "insts_source": src}
 
     result = {
         "metric": "key-switches/sec (N=2^16, L=24, dnum=3, w=60), batched fused form; its hipGraph replay, the "
@@ -238,12 +285,27 @@ def main():
                    "keyswitch_fused_bytes": batched.fused_bytes() // B,
                    "fused_tuned_shape": batched.fused.config()},
         "roofline": {"bound": "hbm",
-                     "kernel": f"k_ntt_pass_r16<8,false>: forward NTT of {limbs_fwd} limbs of N={N}, "
-                               f"{n_launch} launches per NTT",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4),
-                     "launch_read_write_GBps": round(2 * achieved, 1)},
+                     "kernel": f"k_ks_inner<8> (K2, the dominant kernel of the fused key-switch): {B} key-switches "
+                               f"of N={N}, L={L}, dnum={D}",
+                     "achieved": round(k2_bytes / k2_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(k2_bytes / k2_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": k2_traffic,
+                     "traffic_source": k2_traffic_src,
+                     "bytes_per_launch": k2_bytes, "avg_launch_ms": round(k_ms[1], 4),
+                     "kernel_ms": {"K1": round(k_ms[0], 4), "K2": round(k_ms[1], 4), "K3": round(k_ms[2], 4)},
+                     "marked_step_ms": round(t_marked / args.steps * 1e3, 4),
+                     "valu": valu_entry(k2_valu, k2_valu_src, k2_s),
+                     "keyswitch_GBps": round(batched.algorithmic_bytes() * args.steps / t_fused / 1e9, 1),
+                     "keyswitch_frac": round(batched.algorithmic_bytes() * args.steps / t_fused / 1e9 / HBM_PEAK_GBS,
+                                             4)},
+        "roofline_ntt_pass": {"bound": "hbm",
+                              "kernel": f"k_ntt_pass_r16<8,false>: forward NTT of {limbs_fwd} limbs of N={N}, "
+                                        f"{n_launch} launches per NTT",
+                              "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                              "traffic_source": profiled("ntt_pass_traffic")[1],
+                              "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4),
+                              "launch_read_write_GBps": round(2 * achieved, 1),
+                              "valu": valu_entry(ntt_valu, ntt_valu_src, avg_launch)},
         "cpu_baseline": None,
     }
     try:

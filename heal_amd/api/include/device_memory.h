@@ -43,7 +43,7 @@ struct DeviceTensor {
     std::shared_ptr<lattica_hw_api::detail::Buffer> buffer;
     int64_t offset = 0;  // elements from the buffer start (views made by get_slice)
 
-    // reference semantics (example_impl/device_memory_impl.cpp:78-120): the new
+    // reference semantics (example_impl/device_memory_impl.cpp:42-84): the new
     // shape must have as many elements as the dims with non-zero stride; the
     // tensor gets C-contiguous strides for the new shape
     void reshape(const std::vector<int64_t>& new_dims);

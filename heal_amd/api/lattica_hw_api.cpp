@@ -266,7 +266,7 @@ size_t captured_plan_count() { return (size_t)g_pinned.load(); }
 
 // ------------------------------------------------------------------ memory
 
-// allocate_on_hardware (example_impl/device_memory_impl.cpp:204-215): zeroed, C-contiguous
+// allocate_on_hardware (example_impl/device_memory_impl.cpp:168-179): zeroed, C-contiguous
 template <typename T>
 std::shared_ptr<DeviceTensor<T>> allocate_on_hardware(const std::vector<int64_t>& dims) {
     for (int64_t d : dims)
@@ -279,7 +279,7 @@ std::shared_ptr<DeviceTensor<T>> allocate_on_hardware(const std::vector<int64_t>
     return t;
 }
 
-// host_to_device (example_impl/device_memory_impl.cpp:217-226): dtype must be T;
+// host_to_device (example_impl/device_memory_impl.cpp:181-190): dtype must be T;
 // the tensor's strided span is copied and its strides are kept
 template <typename T>
 std::shared_ptr<DeviceTensor<T>> host_to_device(const torch::Tensor& tensor) {
@@ -300,7 +300,7 @@ std::shared_ptr<DeviceTensor<T>> host_to_device(const torch::Tensor& tensor) {
     return t;
 }
 
-// device_to_host (example_impl/device_memory_impl.cpp:228-238): a host tensor
+// device_to_host (example_impl/device_memory_impl.cpp:192-202): a host tensor
 // with the device tensor's dims and strides, cloned
 template <typename T>
 torch::Tensor device_to_host(const std::shared_ptr<DeviceTensor<T>>& memory) {

@@ -95,6 +95,8 @@ _PROTOS = {
     "hx_keyswitch_prepare_key": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
     "hx_keyswitch_exec": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, VP, STREAM],
     "hx_keyswitch_plan_destroy": [ctypes.c_void_p],
+    "hx_keyswitch_exec_marked": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, VP,
+                                 ctypes.POINTER(ctypes.c_void_p), STREAM],
     "hx_keyswitch_exec_ex": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, I64, I64, ctypes.c_void_p, VP,
                              STREAM],
     "hx_keyswitch_row_map": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
@@ -421,6 +423,15 @@ class KeySwitchPlan:
         _need_cuda(x, key, out)
         check(lib.hx_keyswitch_exec(self._h, view(x), key.data_ptr(), key.numel() * key.element_size(), view(out),
                                     stream_of(out)))
+        return out
+
+    def exec_marked(self, x, key, out, events):
+        """The key-switch with torch.cuda.Events recorded before the first kernel
+        and after each of the three (events: four recorded-once Events or None)."""
+        _need_cuda(x, key, out)
+        ev = (ctypes.c_void_p * 4)(*[e.cuda_event if e is not None else None for e in events])
+        check(lib.hx_keyswitch_exec_marked(self._h, view(x), key.data_ptr(), key.numel() * key.element_size(),
+                                           view(out), ev, stream_of(out)))
         return out
 
     def exec_ex(self, x, key, out, key_limbs: int, key_count: int, row_map=None):

@@ -62,6 +62,8 @@ struct InnerArgs {
     int64_t m, k, B;
     int32_t logm, hhi, D, G;
     int64_t n_tg, n_hi;
+    int32_t H;            // the inner window (= logm - hhi)
+    int32_t tpt, per;     // grouped launches: threads per tile, tiles per workgroup
 };
 
 // LDS rows of K2 (64-bit slots): one pad slot every 16 and one more per row,
@@ -170,15 +172,16 @@ __device__ __forceinline__ void ks_dot(uint64_t (&x)[16], const uint64_t* xf, co
 //            store on thread s + NI*lane (adjacent columns again).
 // The prepared key holds window index wB(e, lane) of row block hi at
 // hi*W + e*TL + lane.
+// Tile `blk` of `nblk`, served by threads tid = 0.. of a workgroup with an
+// LDS block lds of [max(NF, NI)][SP] slots.  A thread whose tid is beyond the
+// tile's threads stays idle but runs the same barriers.
 template <int H>
-__device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_t nblk) {
+__device__ __forceinline__ void ks_tile(const InnerArgs& A, int64_t blk, int64_t nblk, int tid, uint64_t* lds) {
     constexpr int W = 1 << H;
     constexpr int TL = W / 16;
     constexpr int SP = ks_lds_row<H>();
-    extern __shared__ uint64_t lds[];  // [max(NF, NI)][SP]
     const int D = A.D, G = A.G;
     const int NF = G * D, NI = 2 * G;
-    const int tid = threadIdx.x;
 
     // instances vary fastest, then limb groups: the workgroups of one XCD share
     // their twiddles (same limbs and rows) and the partial cache lines at the
@@ -312,14 +315,31 @@ __device__ __forceinline__ void ks_inner(const InnerArgs& A, int64_t blk, int64_
 
 template <int H>
 __global__ __launch_bounds__(512) void k_ks_inner(InnerArgs A) {
-    ks_inner<H>(A, blockIdx.x, gridDim.x);
+    extern __shared__ uint64_t lds[];  // [max(NF, NI)][SP]
+    ks_tile<H>(A, blockIdx.x, gridDim.x, threadIdx.x, lds);
 }
 
-// K2 of several key-switches with the same inner window H in one launch
-template <int H>
-__global__ __launch_bounds__(512) void k_ks_inner_multi(const InnerArgs* list, const int64_t* starts, int count) {
+// K2 of several key-switches of any inner window in one launch: workgroup b
+// serves key-switch list[i] with b' = b - starts[i] and runs its tiles
+// b' * per .. b' * per + per - 1 as `per` groups of `tpt` threads (members of
+// smaller windows pack more tiles into a workgroup); threads beyond them, and
+// tiles beyond the member's, stay idle but run every barrier.
+__global__ __launch_bounds__(512, 3) void k_ks_inner_group(const InnerArgs* list, const int64_t* starts, int count) {
+    extern __shared__ uint64_t lds[];
     const int i = instance_of(starts, count, blockIdx.x);
-    ks_inner<H>(list[i], (int64_t)blockIdx.x - starts[i], starts[i + 1] - starts[i]);
+    const InnerArgs& A = list[i];
+    const int64_t ntiles = A.n_hi * A.n_tg * A.B;
+    const int sub = threadIdx.x / A.tpt;
+    const int64_t tile = ((int64_t)blockIdx.x - starts[i]) * A.per + sub;
+    const bool active = sub < A.per && tile < ntiles;
+    const int tid = active ? (int)threadIdx.x % A.tpt : (1 << 30);
+    const int rows = A.D * A.G > 2 * A.G ? A.D * A.G : 2 * A.G;
+    const int64_t t0 = active ? tile : 0;
+    const int H = A.H;
+    uint64_t* base = lds + (int64_t)(active ? sub : 0) * rows * ks_lds_row_of(H);
+    if (H == 8) ks_tile<8>(A, t0, ntiles, tid, base);
+    else if (H == 7) ks_tile<7>(A, t0, ntiles, tid, base);
+    else ks_tile<6>(A, t0, ntiles, tid, base);
 }
 
 // key[((d*B + b)*2 + c)*k + t][hi*W + e*TL + lane] = evk[d, b, c, rowmap[j], t] mod q_t
@@ -732,6 +752,7 @@ bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void
     A.G = G;
     A.n_tg = (k + G - 1) / G;
     A.n_hi = int64_t(1) << plan->hhi;
+    A.H = plan->hlo;
     K.H2 = plan->hlo;
     inner_shape(K.H2, D, G, &K.threads2, &K.lds2);
     if (K.threads2 > 512 || K.lds2 > 64 * 1024)
@@ -779,21 +800,31 @@ static void launch_inner(const KsArgs& K, hipStream_t st) {
     check_hip(hipGetLastError(), "keyswitch inner launch");
 }
 
-static void keyswitch_launch(KsArgs& K, hipStream_t st) {
+// marks (may be null): events recorded before K1, after K1, after K2 and after K3
+static void keyswitch_launch(KsArgs& K, hipStream_t st, void* const* marks = nullptr) {
+    auto mark = [&](int i) {
+        if (marks && marks[i]) check_hip(hipEventRecord((hipEvent_t)marks[i], st), "hipEventRecord");
+    };
+    mark(0);
     launch_pass(K.P1, st);
+    mark(1);
     launch_inner(K, st);
+    mark(2);
     launch_pass(K.P3, st);
+    mark(3);
 }
 
 void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
-                    int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st) {
+                    int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st,
+                    void* const* marks = nullptr) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(x, "x", 3);
     if (x->dims[0] == 0) return;
     std::lock_guard<std::mutex> lk(plan->wss->mu);
     const Workspaces::Ws w = plan->wss->acquire(st, 0, keyswitch_ws_bytes(plan, x->dims[0]));
     KsArgs K;
-    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, w.data, K)) keyswitch_launch(K, st);
+    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, w.data, K))
+        keyswitch_launch(K, st, marks);
     plan->wss->note(st);
 }
 
@@ -1036,43 +1067,56 @@ hx_keyswitch_group* keyswitch_group_create(const hx_keyswitch_plan* const* plans
         drop();
         throw;
     }
-    // host image: per launch [args...][starts...]
+    // host image: per launch [args...][starts...]; one launch per phase for
+    // every member whatever its window sizes
     std::vector<char> image;
     auto align = [&](size_t a) { image.resize((image.size() + a - 1) / a * a); };
-    for (int phase = 1; phase <= 3; ++phase) {
-        for (int H = 6; H <= 8; ++H) {
-            std::vector<const KsArgs*> sel;
-            for (const KsArgs& K : ks)
-                if ((phase == 2 ? K.H2 : K.H1) == H) sel.push_back(&K);
-            if (sel.empty()) continue;
-            GroupLaunch L;
-            L.phase = phase;
-            L.H = H;
-            L.count = (int)sel.size();
-            L.threads = 0;
-            L.lds = 0;
-            align(256);
-            L.args_off = image.size();
-            const size_t asz = phase == 2 ? sizeof(InnerArgs) : sizeof(PassArgs);
-            image.resize(image.size() + asz * sel.size());
-            std::vector<int64_t> starts(1, 0);
-            for (size_t i = 0; i < sel.size(); ++i) {
-                const KsArgs& K = *sel[i];
-                const void* src = phase == 1 ? (const void*)&K.P1 : (phase == 2 ? (const void*)&K.A : (const void*)&K.P3);
-                std::memcpy(image.data() + L.args_off + i * asz, src, asz);
-                starts.push_back(starts.back() + (phase == 1 ? K.blocks1 : (phase == 2 ? K.blocks2 : K.blocks3)));
-                if (phase == 2) {
-                    L.threads = std::max(L.threads, K.threads2);
-                    L.lds = std::max(L.lds, K.lds2);
-                }
+    for (int phase = 1; phase <= 3 && !ks.empty(); ++phase) {
+        GroupLaunch L;
+        L.phase = phase;
+        L.H = 0;
+        L.count = (int)ks.size();
+        L.threads = phase == 2 ? 0 : 512;
+        L.lds = 0;
+        align(256);
+        L.args_off = image.size();
+        const size_t asz = phase == 2 ? sizeof(InnerArgs) : sizeof(PassArgs);
+        image.resize(image.size() + asz * ks.size());
+        std::vector<int64_t> starts(1, 0);
+        // K2: every member packs as many tiles into a workgroup as the widest
+        // member's threads allow (and 64 KB of LDS)
+        int k2_threads = 64;
+        if (phase == 2)
+            for (const KsArgs& K : ks) k2_threads = std::max(k2_threads, K.threads2);
+        if (phase == 2) L.threads = k2_threads;
+        for (size_t i = 0; i < ks.size(); ++i) {
+            KsArgs K = ks[i];
+            int64_t wgs;
+            if (phase == 2) {
+                const int TL = (1 << K.H2) / 16;
+                const int rows = std::max((int)K.A.D * K.A.G, 2 * K.A.G);
+                const size_t tile_lds = (size_t)rows * ks_lds_row_of(K.H2) * 8;
+                K.A.tpt = rows * TL;
+                K.A.per = std::max(1, k2_threads / K.A.tpt);
+                while (K.A.per > 1 && (size_t)K.A.per * tile_lds > 64 * 1024) --K.A.per;
+                wgs = (K.blocks2 + K.A.per - 1) / K.A.per;
+                L.lds = std::max(L.lds, (size_t)K.A.per * tile_lds);
+            } else {
+                const int H = phase == 1 ? K.P1.H : K.P3.H;
+                const int per = 1 << (8 - H);
+                wgs = ((phase == 1 ? K.blocks1 : K.blocks3) + per - 1) / per;
+                L.lds = std::max(L.lds, (size_t)per * kR16Tile * ((1 << H) + 1) * 8);
             }
-            L.blocks = starts.back();
-            align(256);
-            L.starts_off = image.size();
-            image.resize(image.size() + starts.size() * sizeof(int64_t));
-            std::memcpy(image.data() + L.starts_off, starts.data(), starts.size() * sizeof(int64_t));
-            g->launches.push_back(L);
+            const void* src = phase == 1 ? (const void*)&K.P1 : (phase == 2 ? (const void*)&K.A : (const void*)&K.P3);
+            std::memcpy(image.data() + L.args_off + i * asz, src, asz);
+            starts.push_back(starts.back() + wgs);
         }
+        L.blocks = starts.back();
+        align(256);
+        L.starts_off = image.size();
+        image.resize(image.size() + starts.size() * sizeof(int64_t));
+        std::memcpy(image.data() + L.starts_off, starts.data(), starts.size() * sizeof(int64_t));
+        g->launches.push_back(L);
     }
     try {
         check_hip(hipMallocAsync(&g->mem, std::max<size_t>(image.size(), 256), st), "hipMallocAsync(group)");
@@ -1094,16 +1138,19 @@ void keyswitch_group_exec(const hx_keyswitch_group* g, hipStream_t st) {
     for (const GroupLaunch& L : g->launches) {
         const char* base = (const char*)g->mem;
         const int64_t* starts = (const int64_t*)(base + L.starts_off);
+        if (L.blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "too many tiles for one launch");
+        const dim3 grid((unsigned)L.blocks), block((unsigned)L.threads);
         if (L.phase == 2) {
-            const InnerArgs* list = (const InnerArgs*)(base + L.args_off);
-            const dim3 grid((unsigned)L.blocks), block((unsigned)L.threads);
-            if (L.H == 8) hipLaunchKernelGGL(k_ks_inner_multi<8>, grid, block, L.lds, st, list, starts, L.count);
-            else if (L.H == 7) hipLaunchKernelGGL(k_ks_inner_multi<7>, grid, block, L.lds, st, list, starts, L.count);
-            else hipLaunchKernelGGL(k_ks_inner_multi<6>, grid, block, L.lds, st, list, starts, L.count);
-            check_hip(hipGetLastError(), "grouped keyswitch inner launch");
+            hipLaunchKernelGGL(k_ks_inner_group, grid, block, L.lds, st, (const InnerArgs*)(base + L.args_off), starts,
+                               L.count);
+        } else if (L.phase == 3) {
+            hipLaunchKernelGGL(k_ntt_pass_r16_group<true>, grid, block, L.lds, st,
+                               (const PassArgs*)(base + L.args_off), starts, L.count);
         } else {
-            launch_pass_multi(L.H, L.phase == 3, (const PassArgs*)(base + L.args_off), starts, L.count, L.blocks, st);
+            hipLaunchKernelGGL(k_ntt_pass_r16_group<false>, grid, block, L.lds, st,
+                               (const PassArgs*)(base + L.args_off), starts, L.count);
         }
+        check_hip(hipGetLastError(), "grouped keyswitch launch");
     }
     const_cast<hx_keyswitch_group*>(g)->uses.note(st);
 }
@@ -1162,6 +1209,14 @@ int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const voi
                       const hx_view* out, hx_stream stream) {
     HX_API_BEGIN
     keyswitch_exec(plan, x, key, key_bytes, -1, -1, nullptr, out, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_exec_marked(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                             const hx_view* out, void* const* events, hx_stream stream) {
+    HX_API_BEGIN
+    if (!events) raise(HX_ERR_INVALID_ARGUMENT, "events pointer must not be null.");
+    keyswitch_exec(plan, x, key, key_bytes, -1, -1, nullptr, out, as_stream(stream), events);
     HX_API_END
 }
 

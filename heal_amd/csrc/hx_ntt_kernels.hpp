@@ -281,40 +281,51 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return x * per + (x < rem ? x : rem) + i;
 }
 
-// Per-tile bookkeeping shared by the pass kernels (one entry per sub-problem).
-struct TileShared {
-    int64_t src_off[64];
-    int64_t dst_off[64];
-    int64_t alt_off[64];
-    int64_t jbase[64];
-    int64_t col[64];
-    int32_t limb[64];
-    int32_t digit[64];
-    int32_t valid[64];
-    int32_t bad[64];   // flag_set: a value of this sub-problem is above the flag limit
-    int32_t skip[64];  // flag_skip: the column was flagged by an earlier pass
+// Per-tile bookkeeping shared by the pass kernels (one entry per sub-problem;
+// N = 64 for the generic pass, 32 for the radix-16 pass).
+template <int N>
+struct TileSharedN {
+    int64_t src_off[N];
+    int64_t dst_off[N];
+    int64_t alt_off[N];
+    int64_t jbase[N];
+    int64_t col[N];
+    int32_t limb[N];
+    int32_t digit[N];
+    int32_t valid[N];
+    int32_t bad[N];   // flag_set: a value of this sub-problem is above the flag limit
+    int32_t skip[N];  // flag_skip: the column was flagged by an earlier pass
 };
+using TileShared = TileSharedN<64>;
 
 // The first pass found a value of column `col` above the flag limit.  In a
 // single pass the sub-problem is the whole column, so its own result is
 // withheld too (bad); stores of the same value from several threads are benign.
-__device__ __forceinline__ void mark_flagged(const PassArgs& P, TileShared& T, int s) {
+template <int N>
+__device__ __forceinline__ void mark_flagged(const PassArgs& P, TileSharedN<N>& T, int s) {
     T.bad[s] = 1;
     P.colflag[T.col[s]] = 1u;
 }
 
-__device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared& T, int64_t blk, int64_t nblk) {
-    int64_t tile = xcd_tile(blk, nblk);
+// Fills T for tile `blk` of `nblk` (a tile index at or beyond nblk gives a
+// tile of no valid sub-problem); tid is the thread's index within the threads
+// that serve the tile.  Returns the index bits above the window, the same for
+// every sub-problem of the tile.
+template <int N>
+__device__ __forceinline__ int64_t tile_setup(const PassArgs& P, int S, TileSharedN<N>& T, int64_t blk, int64_t nblk,
+                                              int tid) {
+    const bool in_range = blk < nblk;
+    int64_t tile = in_range ? xcd_tile(blk, nblk) : 0;
     const int64_t col_tile = tile % P.n_col_tiles;
     tile /= P.n_col_tiles;
     const int64_t bot_tile = tile % P.n_bot_tiles;
     const int64_t top = tile / P.n_bot_tiles;
-    if (threadIdx.x < S) {
-        int s = threadIdx.x;
+    if (tid < S) {
+        int s = tid;
         int cs = s % P.S_cols, bs = s / P.S_cols;
         int64_t col = col_tile * P.S_cols + cs;
         int64_t bot = bot_tile * P.S_bot + bs;
-        int valid = col < P.ncols && bot < P.bot_count;
+        int valid = in_range && col < P.ncols && bot < P.bot_count;
         int64_t c = valid ? col : 0;
         int64_t coord[3];
         coord[0] = c % P.cdim[0];
@@ -335,6 +346,7 @@ __device__ __forceinline__ void tile_setup(const PassArgs& P, int S, TileShared&
         T.skip[s] = (P.colflag && P.flag_skip && valid) ? (int32_t)P.colflag[col] : 0;
     }
     __syncthreads();
+    return top;
 }
 
 // Coalesced load of the tile into LDS: consecutive threads take consecutive
@@ -392,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs P) {
     const int WP = W + 1;
     const int S = P.S_cols * P.S_bot;
     auto slot = [WP](int s, int w) { return s * WP + w; };
-    tile_setup(P, S, T, blockIdx.x, gridDim.x);
+    tile_setup(P, S, T, blockIdx.x, gridDim.x, threadIdx.x);
     tile_load(P, S, W, T, xs, slot);
 
     const int half_count = W >> 1;
@@ -677,23 +689,22 @@ struct R16Lds {
     __device__ static constexpr int slot(int w) { return LM ? w + (w >> 4) : w; }
 };
 
-// One workgroup of a radix-16 pass: tile `blk` of the pass's `nblk` tiles.
+// Tile `blk` of a radix-16 pass's `nblk` tiles, served by the 32 * 2^(H-4)
+// threads tid = 0.. of a workgroup with tile bookkeeping T and an LDS block xs
+// of [32][SP] slots.  Every thread of the workgroup runs the same barriers.
 template <int H, bool INV, bool LM = false>
-__device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
+__device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t nblk, int tid,
+                                         TileSharedN<32>& T, uint64_t* xs) {
     constexpr int W = 1 << H;
-    constexpr int S = 32;               // sub-problems per workgroup
+    constexpr int S = 32;               // sub-problems per tile
     constexpr int SP = R16Lds<H, LM>::SP;
     constexpr int L_IN = INV ? 1 : 0;   // layout of the first stages
     constexpr int L_OUT = 1 - L_IN;
-    extern __shared__ uint64_t xs[];    // [S][SP]
-    __shared__ TileShared T;
-    tile_setup(P, S, T, blk, nblk);
-    // the bits above the window are the same for the whole tile
-    const int64_t top = (xcd_tile(blk, nblk) / P.n_col_tiles) / P.n_bot_tiles;
+    const int64_t top = tile_setup(P, S, T, blk, nblk, tid);
 
     constexpr int TL = W / 16;
-    const int s = LM ? threadIdx.x / TL : threadIdx.x % S;
-    const int lane = LM ? threadIdx.x % TL : threadIdx.x / S;
+    const int s = LM ? tid / TL : tid % S;
+    const int lane = LM ? tid % TL : tid / S;
     const bool valid = T.valid[s];
     const int limb = T.limb[s];
     const LimbInfo& L = P.limbs[limb];
@@ -710,7 +721,7 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
                            (int64_t)r16_wl<H>(L_IN, lane) * wstride;
         if (P.src.kind == 3) {
             // digit d of x: the reference's unsigned digit loop
-            // (example_impl/g_decomposition_impl.cpp:145-149) in closed form
+            // (example_impl/g_decomposition_impl.cpp:52-56) in closed form
             const int d = T.digit[s];
             const int wb = P.gd_w;
             const uint64_t mask = wb ? (~0ull >> (64 - wb)) : 0;
@@ -846,6 +857,14 @@ __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t
     }
 }
 
+// One workgroup of a radix-16 pass: tile `blk` of the pass's `nblk` tiles.
+template <int H, bool INV, bool LM = false>
+__device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
+    extern __shared__ uint64_t xs[];    // [32][SP]
+    __shared__ TileSharedN<32> T;
+    r16_tile<H, INV, LM>(P, blk, nblk, threadIdx.x, T, xs);
+}
+
 template <int H, bool INV, bool LM>
 __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
     r16_pass<H, INV, LM>(P, blockIdx.x, gridDim.x);
@@ -973,13 +992,31 @@ __device__ __forceinline__ int instance_of(const int64_t* starts, int count, int
     return lo;
 }
 
-// Several passes of one window size in one launch: workgroup b runs tile
-// b - starts[i] of pass list[i] (starts has count + 1 entries).
-template <int H, bool INV>
-__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16_multi(const PassArgs* list, const int64_t* starts,
-                                                                       int count) {
+// Tiles of a radix-16 pass (PassArgs prepared by configure_tile / pass_blocks).
+__device__ __forceinline__ int64_t r16_tiles(const PassArgs& P) {
+    return (int64_t(1) << (P.logm - P.b0 - P.H)) * P.n_bot_tiles * P.n_col_tiles;
+}
+
+// Passes of any window size H = 6..8 (one direction) in one launch of 512
+// threads: workgroup b serves pass list[i] with b' = b - starts[i], and runs
+// its 2^(8-H) tiles b' * 2^(8-H) .. as many groups of 32 * 2^(H-4) threads, so
+// that every workgroup uses the same threads and about the same LDS
+// (2^(8-H) * 32 * (2^H + 1) slots) whatever its pass.
+template <bool INV>
+__global__ __launch_bounds__(512, 4) void k_ntt_pass_r16_group(const PassArgs* list, const int64_t* starts,
+                                                               int count) {
+    extern __shared__ uint64_t xs[];
+    __shared__ TileSharedN<32> T[4];
     const int i = instance_of(starts, count, blockIdx.x);
-    r16_pass<H, INV>(list[i], (int64_t)blockIdx.x - starts[i], starts[i + 1] - starts[i]);
+    const int64_t b = (int64_t)blockIdx.x - starts[i];
+    const PassArgs& P = list[i];
+    const int64_t ntiles = r16_tiles(P);
+    const int H = P.H;
+    const int sub = threadIdx.x >> (H + 1), tid = threadIdx.x & ((1 << (H + 1)) - 1);
+    const int64_t tile = (b << (8 - H)) + sub;
+    if (H == 8) r16_tile<8, INV>(P, tile, ntiles, tid, T[0], xs);
+    else if (H == 7) r16_tile<7, INV>(P, tile, ntiles, tid, T[sub], xs + sub * 32 * ((1 << 7) + 1));
+    else r16_tile<6, INV>(P, tile, ntiles, tid, T[sub], xs + sub * 32 * ((1 << 6) + 1));
 }
 
 // dst[u][col] = src[map[u]][col] (forward gather for non-permutation perms)
@@ -1228,29 +1265,6 @@ int64_t pass_blocks(PassArgs& P) {
     const int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
     if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
     return blocks;
-}
-
-// Radix-16 passes of one window size H (and direction) as one launch over
-// device arrays: list[count] of prepared PassArgs, starts[count + 1] tile offsets.
-void launch_pass_multi(int H, bool inverse, const PassArgs* list, const int64_t* starts, int count, int64_t blocks,
-                       hipStream_t st) {
-    if (!r16_window(H)) raise(HX_ERR_RUNTIME, "internal: grouped passes need radix-16 windows");
-    if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "too many tiles for one launch");
-    const int W = 1 << H;
-    const size_t lds = (size_t)kR16Tile * (W + 1) * 8;
-    const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
-#define HX_R16M(HH)                                                                                              \
-    if (inverse) hipLaunchKernelGGL((k_ntt_pass_r16_multi<HH, true>), grid, block, lds, st, list, starts, count);  \
-    else hipLaunchKernelGGL((k_ntt_pass_r16_multi<HH, false>), grid, block, lds, st, list, starts, count);
-    if (H == 8) {
-        HX_R16M(8)
-    } else if (H == 7) {
-        HX_R16M(7)
-    } else {
-        HX_R16M(6)
-    }
-#undef HX_R16M
-    check_hip(hipGetLastError(), "grouped ntt pass launch");
 }
 
 void launch_pass(PassArgs& P, hipStream_t st) {

@@ -253,6 +253,12 @@ int hx_keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, 
 int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                       const hx_view* out, hx_stream stream);
 int hx_keyswitch_plan_destroy(hx_keyswitch_plan* plan);
+/* hx_keyswitch_exec that also records events[0..3] (hipEvent_t, each may be
+ * NULL) on `stream` before the first kernel and after each of the three, so a
+ * caller can time every kernel of the launch sequence.  No counterpart in the
+ * reference (measurement of the fused key-switch). */
+int hx_keyswitch_exec_marked(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                             const hx_view* out, void* const* events, hx_stream stream);
 
 /* hx_keyswitch_exec for serving many rotations with one plan per (m, k):
  *   key_limbs  limbs the key was prepared with (>= the plan's k; the first k

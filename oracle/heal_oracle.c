@@ -81,7 +81,7 @@ static int64_t offset_of(const hx_view* v, const int64_t* coord) {
     return off;
 }
 
-/* DeviceTensor::at_with_broadcast (example_impl/device_memory_impl.cpp:153-160):
+/* DeviceTensor::at_with_broadcast (example_impl/device_memory_impl.cpp:111-124):
  * right-aligned, size-1 dims index 0, other dims take the result coordinate and
  * must be in bounds (at() throws out_of_range otherwise). Returns -1 if OOB. */
 static int64_t bcast_offset(const hx_view* v, const int64_t* full, int nfull) {

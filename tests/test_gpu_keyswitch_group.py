@@ -96,3 +96,44 @@ def test_group_errors(gpu):
     bad = [torch.empty((1, 2, 4096, 2), dtype=torch.int64, device="cuda"), u.outs[1]]
     with pytest.raises(ValueError):
         capi.KeySwitchGroup(u.members, u.x, u.keys, bad)
+
+
+@pytest.mark.slow
+def test_group_uneven_batch_of_64(gpu):
+    """The uneven batch of 64 key-switches (seed 2: N in {2^13..2^16}, L in [4, 24],
+    dnum 3, w 60) in one launch per phase: every member equals its own fused
+    call, eager and replayed from a hipGraph; sampled members (one of each N)
+    equal the oracle's six-call chain."""
+    from heal_amd.keyswitch import UnevenKeySwitch
+
+    print("[group] 64 mixed", flush=True)
+    u = UnevenKeySwitch(count=64, seed=2)
+    outs = [torch.full_like(o, -1) for o in u.outs]
+    grp = u.group(outs)
+    grp()
+    u.run_plain()
+    torch.cuda.synchronize()
+    for i in range(64):
+        assert torch.equal(outs[i], u.outs[i]), (i, u.shapes[i])
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        grp()
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        grp()
+    for rep in range(2):
+        for o in outs:
+            o.fill_(-7)
+        g.replay()
+        torch.cuda.synchronize()
+        for i in range(64):
+            assert torch.equal(outs[i], u.outs[i]), (rep, i, u.shapes[i])
+    first = {}
+    for i, (n, L) in enumerate(u.shapes):
+        first.setdefault(n, i)
+    for n, i in sorted(first.items()):
+        print(f"[group] oracle member {i}: N={n} L={u.shapes[i][1]}", flush=True)
+        assert torch.equal(outs[i][0].cpu(), _oracle(u, i)), (i, u.shapes[i])
+    grp.close()

@@ -150,3 +150,19 @@ def test_gdecomp_overflow_warning_reported_on_sync(lhw, capfd):
     assert out.tolist() == [[5, 0], [0, 0], [7, 0]]
     err = capfd.readouterr().err
     assert "exceed capacity" in err and "base_bits=8" in err
+
+
+def test_allocate_on_hardware_returns_zeros(lhw):
+    """allocate_on_hardware zero-fills its buffer (example_impl/device_memory_impl.cpp:168-179),
+    also when the stream-ordered pool hands back memory that held other values."""
+    for suffix, dtype in (("32", torch.int32), ("64", torch.int64), ("float64", torch.float64)):
+        for shape in ([1 << 20], [3, 5, 7], [2, 0, 4], []):
+            n = 1
+            for d in shape:
+                n *= d
+            junk = getattr(lhw, f"host_to_device_{suffix}")(torch.full((max(n, 1) + 13,), 7, dtype=dtype))
+            del junk  # its memory returns to the pool
+            t = getattr(lhw, f"allocate_on_hardware_{suffix}")(shape)
+            host = getattr(lhw, f"device_to_host_{suffix}")(t)
+            assert list(host.shape) == shape
+            assert bool((host == 0).all()), (suffix, shape)
