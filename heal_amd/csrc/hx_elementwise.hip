@@ -112,6 +112,60 @@ __global__ __launch_bounds__(kBlock) void k_modop(ElemArgs A) {
     }
 }
 
+// Row fast path of modmul / modsum.  The last iteration dim (length k, the
+// limb axis) has stride 1 in the result and in every tensor operand, and the
+// modulus varies along it only (an LDS table).  A work-item takes a segment of
+// kSeg consecutive elements of one row, so the row's offsets are decoded once
+// per segment.  Operands that are residues (0 <= x < |p|) take a short path —
+// one double-width product and one reduction, or one conditional subtraction
+// — whose result is the residue the reference's `%` gives for them; any other
+// element takes the reference's signed semantics as in k_modop.
+// Segments are two int64 elements moved with one 16-byte access per operand
+// (the host checks that every segment start is 16-byte aligned), so adjacent
+// work-items touch adjacent 16-byte words.
+constexpr int kSeg = 2;
+
+__device__ __forceinline__ uint64_t modop_fast(int op, const ModInfo& mi, int64_t av, int64_t bv) {
+    if (mi.M == 0) return 0;
+    if ((uint64_t)av < mi.M && (uint64_t)bv < mi.M) {
+        if (op == 0) return mulmod(mi, (uint64_t)av, (uint64_t)bv);
+        const uint64_t s2 = (uint64_t)av + (uint64_t)bv;
+        return s2 >= mi.M ? s2 - mi.M : s2;
+    }
+    return (uint64_t)(op == 0 ? cpp_mulmod(mi, av, bv) : cpp_addmod(mi, av, bv));
+}
+
+template <typename T, int OP, int BSRC>
+__global__ __launch_bounds__(kBlock) void k_modop_rows(ElemArgs A, FastDiv segs_fd, uint32_t segs, uint32_t nitems) {
+    __shared__ ModInfo table[kTableMax];
+    const int k = A.p_table_len;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) table[i] = modinfo_of(ld<T>(A.p, (int64_t)i * A.p_table_stride));
+    __syncthreads();
+    const IterSpace& it = A.it;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nitems; idx += gridDim.x * blockDim.x) {
+        const uint32_t row = fdiv(idx, segs_fd), sg = idx - row * segs;
+        int64_t o0 = 0, o1 = 0, o2 = 0;
+        uint32_t n = row;
+        for (int d = it.ndim - 2; d >= 0; --d) {
+            const uint32_t q = fdiv(n, it.fd[d]);
+            const int64_t c = (int64_t)(n - q * it.fd[d].d);
+            o0 += c * it.strides[0][d];
+            o1 += c * it.strides[1][d];
+            o2 += c * it.strides[2][d];
+            n = q;
+        }
+        const int t = (int)sg * kSeg;
+        const longlong2 av = *(const longlong2*)((const int64_t*)A.a + o1 + t);
+        longlong2 bv;
+        if (BSRC == SRC_SCALAR) bv = make_longlong2(A.b_scalar, A.b_scalar);
+        else bv = *(const longlong2*)((const int64_t*)A.b + o2 + t);
+        longlong2 r;
+        r.x = (int64_t)modop_fast(OP, table[t], av.x, bv.x);
+        r.y = (int64_t)modop_fast(OP, table[t + 1], av.y, bv.y);
+        *(longlong2*)((int64_t*)A.out + o0 + t) = r;
+    }
+}
+
 // Operand strides broadcast (right-aligned) onto the result's dims: size-1
 // dims and missing leading dims get stride 0.  Operand dims larger than the
 // result's (allowed for p by the reference's checks) index by the result's
@@ -215,6 +269,34 @@ void modop(int op, const hx_view* a, const hx_view* b, int64_t bs, const hx_view
         }
     }
     bool is32 = result->dtype == HX_INT32;
+    // row fast path (see k_modop_rows)
+    const int last = A.it.ndim - 1;
+    const int64_t k = A.it.dims[last];
+    const int64_t segs = (k + kSeg - 1) / kSeg;
+    const int64_t nitems = A.it.total / (k ? k : 1) * segs;
+    // every segment start must be 16-byte aligned: even row strides and k, aligned bases
+    bool aligned = k % 2 == 0 && (uintptr_t)result->data % 16 == 0 && (uintptr_t)a->data % 16 == 0 &&
+                   (!b || (uintptr_t)b->data % 16 == 0);
+    for (int d = 0; d < last && aligned; ++d)
+        aligned = A.it.strides[0][d] % 2 == 0 && A.it.strides[1][d] % 2 == 0 && (!b || A.it.strides[2][d] % 2 == 0);
+    if (!is32 && aligned && (op == 0 || op == 1) && psrc == SRC_TABLE && A.it.use32 && A.it.strides[0][last] == 1 &&
+        A.it.strides[1][last] == 1 && (!b || A.it.strides[2][last] == 1) && nitems < (int64_t(1) << 31)) {
+        const FastDiv fd = make_fastdiv((uint32_t)segs);
+        const unsigned g = grid_for(nitems, kBlock, 8192);
+#define HX_ROWS(TT, OPV)                                                                                      \
+    if (b) hipLaunchKernelGGL((k_modop_rows<TT, OPV, SRC_TENSOR>), dim3(g), dim3(kBlock), 0, st, A, fd,       \
+                               (uint32_t)segs, (uint32_t)nitems);                                              \
+    else hipLaunchKernelGGL((k_modop_rows<TT, OPV, SRC_SCALAR>), dim3(g), dim3(kBlock), 0, st, A, fd,         \
+                            (uint32_t)segs, (uint32_t)nitems);
+        if (op == 0) {
+            HX_ROWS(int64_t, 0)
+        } else {
+            HX_ROWS(int64_t, 1)
+        }
+#undef HX_ROWS
+        check_hip(hipGetLastError(), "modop rows launch");
+        return;
+    }
     if (op == 0) {
         if (is32) launch_modop_t<int32_t, 0>(A, bsrc, psrc, st);
         else launch_modop_t<int64_t, 0>(A, bsrc, psrc, st);
@@ -651,6 +733,40 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermArgs A) {
     if (bad) atomicAdd(&A.status->index_out_of_range, bad);
 }
 
+// Fast path of permute for a validated table whose rows all equal row 0 (one
+// map for every row, as for a Galois automorphism of every limb) when the
+// elementwise axis is the last dim with stride 1 in both a and the result: a
+// work-item takes kSeg consecutive elements along it, decodes its offsets and
+// loads the permutation entry once, and copies the segment from the source
+// row with one 16-byte access (int64, segment starts 16-byte aligned).
+// Entries outside [0, m) write 0 and are counted, as in k_permute.
+__global__ __launch_bounds__(kBlock) void k_permute_rows(PermArgs A, FastDiv segs_fd, uint32_t segs, uint32_t nitems) {
+    const IterSpace& it = A.it;
+    unsigned long long bad = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nitems; idx += gridDim.x * blockDim.x) {
+        const uint32_t row = fdiv(idx, segs_fd), sg = idx - row * segs;
+        int64_t ooff = 0, aoff = 0, cpa = 0;
+        uint32_t n = row;
+        for (int d = it.ndim - 2; d >= 0; --d) {
+            const uint32_t q = fdiv(n, it.fd[d]);
+            const int64_t c = (int64_t)(n - q * it.fd[d].d);
+            if (d == A.pa_dim) cpa = c;
+            ooff += c * it.strides[0][d];
+            aoff += c * it.strides[1][d];
+            n = q;
+        }
+        const int64_t pi = ld<int64_t>(A.perms, cpa * A.ps1);
+        const bool ok = pi >= 0 && pi < A.m;
+        const int64_t src = aoff + (pi - cpa) * A.a_pa_stride;
+        const int t = (int)sg * kSeg;
+        longlong2 v = make_longlong2(0, 0);
+        if (ok) v = *(const longlong2*)((const int64_t*)A.a + src + t);
+        else bad += 2;
+        *(longlong2*)((int64_t*)A.out + ooff + t) = v;
+    }
+    if (bad) atomicAdd(&A.status->index_out_of_range, bad);
+}
+
 // Entries of rows 1.. of a 2-D table that differ from the same column of row 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perm_rows_diff(const void* perms, int64_t l, int64_t m, int64_t s0,
@@ -787,9 +903,24 @@ void permute(const hx_view* a, const hx_view* perms, const hx_view* result, int6
     A.ps1 = perms->strides[1];
     A.m = m;
     A.status = status;
-    unsigned g = grid_for(it.total, kBlock);
-    if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_permute<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL(k_permute<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    // row fast path (see k_permute_rows)
+    const int last = nd - 1;
+    const int64_t k = a->dims[last];
+    const int64_t segs = (k + kSeg - 1) / kSeg;
+    const int64_t nitems = k ? it.total / k * segs : 0;
+    bool aligned = result->dtype == HX_INT64 && k % 2 == 0 && (uintptr_t)result->data % 16 == 0 &&
+                   (uintptr_t)src.data % 16 == 0;
+    for (int d = 0; d < last && aligned; ++d) aligned = result->strides[d] % 2 == 0 && src.strides[d] % 2 == 0;
+    if (aligned && rows_same && ew == last && pa != last && it.use32 && result->strides[last] == 1 &&
+        src.strides[last] == 1 && nitems < (int64_t(1) << 31)) {
+        const FastDiv fd = make_fastdiv((uint32_t)segs);
+        const unsigned g = grid_for(nitems, kBlock, 8192);
+        hipLaunchKernelGGL(k_permute_rows, dim3(g), dim3(kBlock), 0, st, A, fd, (uint32_t)segs, (uint32_t)nitems);
+    } else {
+        unsigned g = grid_for(it.total, kBlock);
+        if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_permute<int32_t>, dim3(g), dim3(kBlock), 0, st, A);
+        else hipLaunchKernelGGL(k_permute<int64_t>, dim3(g), dim3(kBlock), 0, st, A);
+    }
     check_hip(hipGetLastError(), "permute launch");
     if (tmp) check_hip(hipFreeAsync(tmp, st), "hipFreeAsync");
 }
