@@ -75,6 +75,7 @@ _PROTOS = {
     "hx_modneg_tc": [VP, I64, VP, STREAM],
     "hx_modmul_axis_sum": [VP, VP, VP, VP, I64, STREAM],
     "hx_abs": [VP, VP, STREAM],
+    "hx_rot_mac": [VP, VP, VP, VP, VP, I64, VP, STREAM],
     "hx_set_const": [VP, I64, ctypes.c_double, STREAM],
     "hx_g_decomposition": [VP, VP, U64, U64, STREAM],
     "hx_ntt": [VP, VP, VP, VP, VP, VP, VP, STREAM],
@@ -269,6 +270,13 @@ def modmul_axis_sum(a, b, p, out, axis: int):
     _need_cuda(a, b, p, out)
     check(lib.hx_modmul_axis_sum(view(a), view(b), view(p), view(out), int(axis), stream_of(out)))
     return out
+
+
+def rot_mac(acc, ks, ct, perm, diags, diag: int, p):
+    """One diagonal rotate-multiply-accumulate step (hx_rot_mac); ks=None for diagonal 0."""
+    _need_cuda(acc, ct, diags, p)
+    check(lib.hx_rot_mac(view(acc), view(ks), view(ct), view(perm), view(diags), int(diag), view(p), stream_of(acc)))
+    return acc
 
 
 def abs_(a, out):

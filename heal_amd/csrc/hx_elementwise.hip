@@ -529,6 +529,130 @@ void modmul_axis_sum(const hx_view* a, const hx_view* b, const hx_view* p, const
     check_hip(hipGetLastError(), "modmul_axis_sum launch");
 }
 
+// --------------------------------------------- diagonal rotate-multiply-accumulate
+// One step of the diagonal matrix-vector product on ciphertexts, the calls
+//     rot_0 = modsum_ttt(ks_0, permute(ct_0, perm))   (rot = ct when ks is null)
+//     rot_1 = ks_1
+//     acc_c = modsum_ttt(acc_c, modmul_ttt(rot_c, pt))
+// with pt[b, j, t] = diags[b, diag, j mod M, t] (the matrix stored by
+// diagonals: diags[b, r, i, t] = M_b[i, (i + r) mod M] mod q_t, so that one
+// step reads one contiguous [M, L] block per instance), fused: each
+// work-item takes one (b, j, t) for both polynomials c.  Every step is the
+// reference op's own (modop_fast: the residue fast path, the C++ semantics
+// otherwise), so the result equals the five calls.
+struct RotMacArgs {
+    int64_t* acc;
+    const int64_t* ks;
+    const int64_t* ct;
+    const int64_t* perm;
+    const int64_t* diags;
+    const int64_t* p;
+    int64_t acc_b, ks_b, ct_b, diags_b;  // batch strides
+    int64_t n, L, M, diag;
+    int64_t total;                      // B * n * L
+    FastDiv fd_L, fd_n;
+    DevStatus* status;
+};
+
+__global__ __launch_bounds__(kBlock) void k_rot_mac(RotMacArgs A) {
+    __shared__ ModInfo table[kTableMax];
+    for (int i = threadIdx.x; i < A.L; i += blockDim.x) table[i] = modinfo_of(A.p[i]);
+    __syncthreads();
+    const int64_t plane = A.n * A.L;
+    unsigned long long bad = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)A.total;
+         idx += gridDim.x * blockDim.x) {
+        const uint32_t rest = fdiv(idx, A.fd_L), t = idx - rest * (uint32_t)A.L;
+        const uint32_t b = fdiv(rest, A.fd_n), j = rest - b * (uint32_t)A.n;
+        const ModInfo mi = table[t];
+        const int64_t e = (int64_t)j * A.L + t;
+        const int64_t pt = A.diags[b * A.diags_b + (A.diag * A.M + (j & (A.M - 1))) * A.L + t];
+        int64_t r0, r1;
+        if (A.ks) {
+            const int64_t src = A.perm[j];
+            const bool ok = src >= 0 && src < A.n;
+            bad += !ok;
+            const int64_t c0 = ok ? A.ct[b * A.ct_b + src * A.L + t] : 0;
+            r0 = (int64_t)modop_fast(1, mi, A.ks[b * A.ks_b + e], c0);
+            r1 = A.ks[b * A.ks_b + plane + e];
+        } else {
+            r0 = A.ct[b * A.ct_b + e];
+            r1 = A.ct[b * A.ct_b + plane + e];
+        }
+        int64_t* acc = A.acc + b * A.acc_b + e;
+        acc[0] = (int64_t)modop_fast(1, mi, acc[0], (int64_t)modop_fast(0, mi, r0, pt));
+        acc[plane] = (int64_t)modop_fast(1, mi, acc[plane], (int64_t)modop_fast(0, mi, r1, pt));
+    }
+    if (bad) atomicAdd(&A.status->index_out_of_range, bad);
+}
+
+// [B, 2, n, L] (or [B, M, M, L] for diags) with unit strides inside an instance
+void require_instances(const hx_view* v, const int64_t* dims, const char* name) {
+    check_view(v, name);
+    if (v->dtype != HX_INT64) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must be int64");
+    if (v->ndim != 4) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must be 4-D");
+    int64_t s = 1;
+    for (int d = 3; d >= 0; --d) {
+        if (v->dims[d] != dims[d]) raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " has the wrong shape");
+        if (d > 0 && v->dims[d] > 1 && v->strides[d] != s)
+            raise(HX_ERR_INVALID_ARGUMENT, std::string(name) + " must be contiguous within an instance");
+        s *= v->dims[d];
+    }
+}
+
+void rot_mac(const hx_view* acc, const hx_view* ks, const hx_view* ct, const hx_view* perm, const hx_view* diags,
+             int64_t diag, const hx_view* p, hipStream_t st) {
+    check_view(acc, "acc");
+    check_view(p, "p");
+    if (acc->ndim != 4) raise(HX_ERR_INVALID_ARGUMENT, "acc must have shape [B, 2, n, L]");
+    const int64_t B = acc->dims[0], n = acc->dims[2], L = acc->dims[3];
+    const int64_t shape[4] = {B, 2, n, L};
+    require_instances(acc, shape, "acc");
+    require_instances(ct, shape, "ct");
+    if (ks) require_instances(ks, shape, "ks");
+    if (!diags || diags->ndim != 4) raise(HX_ERR_INVALID_ARGUMENT, "diags must have shape [B, M, M, L]");
+    const int64_t M = diags->dims[1];
+    const int64_t mshape[4] = {B, M, M, L};
+    require_instances(diags, mshape, "diags");
+    if (M <= 0 || (M & (M - 1)) != 0) raise(HX_ERR_INVALID_ARGUMENT, "the matrix size M must be a power of two");
+    if (p->dtype != HX_INT64 || p->ndim != 1 || p->dims[0] != L || (L > 1 && p->strides[0] != 1))
+        raise(HX_ERR_INVALID_ARGUMENT, "p must be a contiguous int64 tensor of shape [L]");
+    if (L > kTableMax) raise(HX_ERR_INVALID_ARGUMENT, "at most " + std::to_string(kTableMax) + " limbs");
+    if (ks) {
+        if (!perm || perm->dtype != HX_INT64 || perm->ndim != 1 || perm->dims[0] != n ||
+            (n > 1 && perm->strides[0] != 1))
+            raise(HX_ERR_INVALID_ARGUMENT, "perm must be a contiguous int64 tensor of shape [n]");
+    }
+    if (diag < 0) raise(HX_ERR_INVALID_ARGUMENT, "diag must be >= 0");
+    const int64_t total = B * n * L;
+    if (total == 0) return;
+    if (total >= (int64_t(1) << 31)) raise(HX_ERR_INVALID_ARGUMENT, "rot_mac: B * n * L must be below 2^31");
+    if (views_overlap(acc, ct) || (ks && views_overlap(acc, ks)) || views_overlap(acc, diags))
+        raise(HX_ERR_INVALID_ARGUMENT, "acc must not overlap the other operands");
+    RotMacArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.acc = (int64_t*)acc->data;
+    A.ks = ks ? (const int64_t*)ks->data : nullptr;
+    A.ct = (const int64_t*)ct->data;
+    A.perm = ks ? (const int64_t*)perm->data : nullptr;
+    A.diags = (const int64_t*)diags->data;
+    A.p = (const int64_t*)p->data;
+    A.acc_b = acc->strides[0];
+    A.ks_b = ks ? ks->strides[0] : 0;
+    A.ct_b = ct->strides[0];
+    A.diags_b = diags->strides[0];
+    A.n = n;
+    A.L = L;
+    A.M = M;
+    A.diag = diag % M;
+    A.total = total;
+    A.fd_L = make_fastdiv((uint32_t)L);
+    A.fd_n = make_fastdiv((uint32_t)n);
+    A.status = device_status();
+    hipLaunchKernelGGL(k_rot_mac, dim3(grid_for(total, kBlock, 8192)), dim3(kBlock), 0, st, A);
+    check_hip(hipGetLastError(), "rot_mac launch");
+}
+
 // ------------------------------------------------------------ abs, fill
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_abs(ElemArgs A) {
@@ -1014,6 +1138,14 @@ int hx_modmul_axis_sum(const hx_view* a, const hx_view* b, const hx_view* p, con
     HX_API_BEGIN
     if (!a || !b || !p || !result) raise(HX_ERR_INVALID_ARGUMENT, "tensor pointers must not be null.");
     hx::modmul_axis_sum(a, b, p, result, axis, as_stream(stream));
+    HX_API_END
+}
+
+int hx_rot_mac(const hx_view* acc, const hx_view* ks, const hx_view* ct, const hx_view* perm, const hx_view* diags,
+               int64_t diag, const hx_view* p, hx_stream stream) {
+    HX_API_BEGIN
+    if (!acc || !ct || !diags || !p) raise(HX_ERR_INVALID_ARGUMENT, "tensor pointers must not be null.");
+    hx::rot_mac(acc, ks, ct, perm, diags, diag, p, as_stream(stream));
     HX_API_END
 }
 

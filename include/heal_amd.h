@@ -140,6 +140,22 @@ int hx_modneg_tc(const hx_view* a, int64_t p, const hx_view* result, hx_stream s
 int hx_modmul_axis_sum(const hx_view* a, const hx_view* b, const hx_view* p, const hx_view* result, int64_t axis,
                        hx_stream stream);
 int hx_abs(const hx_view* a, const hx_view* result, hx_stream stream);
+
+/* One step of the diagonal matrix-vector product on ciphertexts (the serving
+ * loop's rotate-multiply-accumulate; no counterpart in the reference).  With
+ * acc, ct and ks [B, 2, n, L], diags [B, M, M, L] (M a power of two), perm [n]
+ * and p [L], all int64 and contiguous within an instance:
+ *     rot_0 = modsum_ttt(ks[:, 0], permute(ct[:, 0], perm over rows)),  rot_1 = ks[:, 1]
+ *     (rot = ct when ks is NULL; perm is then not read)
+ *     acc[:, c] = modsum_ttt(acc[:, c], modmul_ttt(rot_c, pt)),
+ *     pt[b, j, t] = diags[b, diag mod M, j mod M, t]
+ * in one kernel, each step with the semantics of the op it names (diags holds
+ * a matrix by diagonals: diags[b, r, i, t] = M_b[i, (i + r) mod M] mod q_t).
+ * Entries of perm outside [0, n) read 0 and are counted as in permute
+ * (reported by the next status poll).  acc must not overlap the other
+ * operands; B * n * L < 2^31, L <= 512. */
+int hx_rot_mac(const hx_view* acc, const hx_view* ks, const hx_view* ct, const hx_view* perm, const hx_view* diags,
+               int64_t diag, const hx_view* p, hx_stream stream);
 int hx_set_const(const hx_view* a, int64_t ivalue, double fvalue, hx_stream stream);
 
 /* result[..., d] = d-th base-2^base_bits digit of a (unsigned digits)
