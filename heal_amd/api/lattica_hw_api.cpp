@@ -8,6 +8,7 @@
 // prepared once per (buffer id, write version, view) and reused.
 #include "include/lattica_hw_api.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <functional>
@@ -24,6 +25,7 @@
 #include <string>
 #include <tuple>
 #include <typeinfo>
+#include <vector>
 
 #include "../../include/heal_amd.h"
 
@@ -106,25 +108,54 @@ std::vector<int64_t> contiguous_strides(const std::vector<int64_t>& dims) {
     return s;
 }
 
-// Deferred reporting of data-dependent conditions: a g_decomposition whose
-// values were not exhausted is reported (as the reference's stderr warning)
-// when the stream is next synchronised.
+// Deferred reporting of data-dependent conditions: the elements of a
+// g_decomposition whose values were not exhausted are reported with the
+// reference's stderr warning (example_impl/g_decomposition_impl.cpp:58-65)
+// when the stream is next synchronised.  Each call carries a tag into the
+// device's overflow log; the tag names the call's shape, base_bits and power.
+struct GdCall {
+    std::vector<int64_t> dims;
+    size_t power = 0, base_bits = 0;
+};
 struct PendingWarn {
     std::mutex mu;
-    bool gdecomp = false;
-    size_t power = 0, base_bits = 0;
+    uint64_t next_tag = 0;
+    std::map<uint64_t, GdCall> calls;  // recent calls by tag (the oldest are dropped beyond kKeep)
+    static constexpr size_t kKeep = 4096;
 };
 static PendingWarn g_warn;
 
+static void print_gdecomp_warning(const GdCall& c, uint64_t index) {
+    std::vector<int64_t> coord(c.dims.size());
+    for (size_t i = c.dims.size(); i-- > 0;) {
+        const int64_t d = c.dims[i] > 0 ? c.dims[i] : 1;
+        coord[i] = (int64_t)(index % (uint64_t)d);
+        index /= (uint64_t)d;
+    }
+    std::cerr << "Warning: value at ";
+    for (auto x : coord) std::cerr << x << " ";
+    std::cerr << "exceeds capacity with base_bits=" << c.base_bits << " and power=" << c.power << "\n";
+}
+
 void sync_and_report() {
     hx_device_status st;
-    check(hx_status_poll(t_stream, &st));
+    hx_gdecomp_log log;
+    check(hx_status_poll_log(t_stream, &st, &log));
+    if (!st.gdecomp_overflow) return;
     std::lock_guard<std::mutex> lk(g_warn.mu);
-    if (st.gdecomp_overflow) {
-        std::cerr << "Warning: " << st.gdecomp_overflow << " value(s) exceed capacity with base_bits=" << g_warn.base_bits
-                  << " and power=" << g_warn.power << "\n";
+    std::vector<std::pair<uint64_t, uint64_t>> entries;
+    for (uint64_t i = 0; i < log.count; ++i) entries.emplace_back(log.tag[i], log.index[i]);
+    std::sort(entries.begin(), entries.end());
+    uint64_t printed = 0;
+    for (const auto& e : entries) {
+        auto it = g_warn.calls.find(e.first);
+        if (it == g_warn.calls.end()) continue;
+        print_gdecomp_warning(it->second, e.second);
+        ++printed;
     }
-    g_warn.gdecomp = false;
+    if (st.gdecomp_overflow > printed)
+        std::cerr << "Warning: " << (st.gdecomp_overflow - printed)
+                  << (printed ? " more" : "") << " value(s) exceed capacity in g_decomposition\n";
 }
 
 // ---- caches of prepared device tables, keyed by buffer identity and version
@@ -544,12 +575,15 @@ template <typename T>
 void g_decomposition(const TensorPtr<T>& a, TensorPtr<T>& result, size_t power, size_t base_bits) {
     require(a, "a"), require(result, "result");
     hx_view av = HX_V(a), rv = HX_V(result);
-    check(hx_g_decomposition(&av, &rv, power, base_bits, t_stream));
+    uint64_t tag;
+    {
+        std::lock_guard<std::mutex> lk(g_warn.mu);
+        tag = ++g_warn.next_tag;
+        g_warn.calls[tag] = GdCall{a->dims, power, base_bits};
+        while (g_warn.calls.size() > PendingWarn::kKeep) g_warn.calls.erase(g_warn.calls.begin());
+    }
+    check(hx_g_decomposition_tagged(&av, &rv, power, base_bits, tag, t_stream));
     bump(result);
-    std::lock_guard<std::mutex> lk(g_warn.mu);
-    g_warn.gdecomp = true;
-    g_warn.power = power;
-    g_warn.base_bits = base_bits;
 }
 
 // ntt / intt: the reference validates a, result, p, perm and twiddles in this

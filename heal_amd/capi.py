@@ -42,6 +42,14 @@ class HxDeviceStatus(ctypes.Structure):
     _fields_ = [("gdecomp_overflow", ctypes.c_uint64), ("index_out_of_range", ctypes.c_uint64)]
 
 
+HX_GDECOMP_LOG_MAX = 64
+
+
+class HxGdecompLog(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("tag", ctypes.c_uint64 * HX_GDECOMP_LOG_MAX),
+                ("index", ctypes.c_uint64 * HX_GDECOMP_LOG_MAX)]
+
+
 VP = ctypes.POINTER(HxView)
 I64 = ctypes.c_int64
 U64 = ctypes.c_uint64
@@ -50,6 +58,7 @@ STREAM = ctypes.c_void_p
 # name -> argtypes (all return int status)
 _PROTOS = {
     "hx_status_poll": [STREAM, ctypes.POINTER(HxDeviceStatus)],
+    "hx_status_poll_log": [STREAM, ctypes.POINTER(HxDeviceStatus), ctypes.POINTER(HxGdecompLog)],
     "hx_malloc": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, STREAM],
     "hx_free": [ctypes.c_void_p, STREAM],
     "hx_memset_zero": [ctypes.c_void_p, ctypes.c_size_t, STREAM],
@@ -78,6 +87,7 @@ _PROTOS = {
     "hx_rot_mac": [VP, VP, VP, VP, VP, I64, VP, STREAM],
     "hx_set_const": [VP, I64, ctypes.c_double, STREAM],
     "hx_g_decomposition": [VP, VP, U64, U64, STREAM],
+    "hx_g_decomposition_tagged": [VP, VP, U64, U64, U64, STREAM],
     "hx_ntt": [VP, VP, VP, VP, VP, VP, VP, STREAM],
     "hx_intt": [VP, VP, VP, VP, VP, VP, VP, VP, STREAM],
     "hx_permute": [VP, VP, VP, I64, I64, STREAM],
@@ -294,9 +304,14 @@ def set_const(a, value):
     return a
 
 
-def g_decomposition(a, out, power: int, base_bits: int):
+def g_decomposition(a, out, power: int, base_bits: int, tag: int = 0):
+    """tag (optional): carried by the elements of this call in the overflow log (status_poll)."""
     _need_cuda(a, out)
-    check(lib.hx_g_decomposition(view(a), view(out), int(power), int(base_bits), stream_of(out)))
+    if tag:
+        check(lib.hx_g_decomposition_tagged(view(a), view(out), int(power), int(base_bits), int(tag),
+                                            stream_of(out)))
+    else:
+        check(lib.hx_g_decomposition(view(a), view(out), int(power), int(base_bits), stream_of(out)))
     return out
 
 
@@ -347,9 +362,14 @@ def stream_capturing(stream: int | None = None) -> bool:
 
 
 def status_poll(stream: int | None = None) -> dict:
+    """Takes the device counters and the g_decomposition overflow log:
+    gdecomp_log lists (tag, row-major element index) pairs, sorted."""
     st = HxDeviceStatus()
-    check(lib.hx_status_poll(stream if stream is not None else stream_of(), ctypes.byref(st)))
-    return {"gdecomp_overflow": st.gdecomp_overflow, "index_out_of_range": st.index_out_of_range}
+    log = HxGdecompLog()
+    check(lib.hx_status_poll_log(stream if stream is not None else stream_of(), ctypes.byref(st), ctypes.byref(log)))
+    entries = sorted((int(log.tag[i]), int(log.index[i])) for i in range(log.count))
+    return {"gdecomp_overflow": st.gdecomp_overflow, "index_out_of_range": st.index_out_of_range,
+            "gdecomp_log": entries}
 
 
 class NttPlan:

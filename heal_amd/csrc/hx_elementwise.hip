@@ -716,13 +716,14 @@ void set_const(const hx_view* a, int64_t ivalue, double fvalue, hipStream_t st) 
 
 // -------------------------------------------------------- g_decomposition
 struct GdArgs {
-    IterSpace it;  // over a: operand 0 = result (digit 0), 1 = a
+    IterSpace it;  // over a: operand 0 = result (digit 0), 1 = a, 2 = row-major index of the element
     void* out;
     const void* a;
     int64_t digit_stride;
     uint32_t power;
     uint32_t base_bits;
     DevStatus* status;
+    unsigned long long tag;  // the call's tag in the overflow log
 };
 
 template <typename T>
@@ -731,15 +732,18 @@ __global__ __launch_bounds__(kBlock) void k_gdecomp(GdArgs A) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     unsigned long long over = 0;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < A.it.total; idx += stride) {
-        int64_t off[2], last;
-        iter_offsets<2>(A.it, idx, off, &last);
+        int64_t off[3], last;
+        iter_offsets<3>(A.it, idx, off, &last);
         int64_t v = ld<T>(A.a, off[1]);
         for (uint32_t d = 0; d < A.power; ++d) {
             uint64_t u = (uint64_t)v;  // size_t arithmetic of the reference
             st_<T>(A.out, off[0] + (int64_t)d * A.digit_stride, to_t<T>((int64_t)(u & mask)));
             v = to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u));
         }
-        if (v > 0) ++over;
+        if (v > 0) {
+            if (!over) gd_log_element(A.status, A.tag, (unsigned long long)off[2]);
+            ++over;
+        }
     }
     if (over) atomicAdd(&A.status->gdecomp_overflow, over);
 }
@@ -757,8 +761,8 @@ __global__ __launch_bounds__(kBlock) void k_gdecomp_digits(GdArgs A, FastDiv pw)
     unsigned long long over = 0;
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const uint32_t e = fdiv(idx, pw), d = idx - e * A.power;
-        int64_t off[2], last;
-        iter_offsets<2>(A.it, (int64_t)e, off, &last);
+        int64_t off[3], last;
+        iter_offsets<3>(A.it, (int64_t)e, off, &last);
         int64_t v = ld<T>(A.a, off[1]);
         for (uint32_t i = 0; i < d; ++i) {
             const uint64_t u = (uint64_t)v;  // size_t arithmetic of the reference
@@ -766,13 +770,53 @@ __global__ __launch_bounds__(kBlock) void k_gdecomp_digits(GdArgs A, FastDiv pw)
         }
         const uint64_t u = (uint64_t)v;
         st_<T>(A.out, off[0] + d, to_t<T>((int64_t)(u & mask)));
-        if (d + 1 == A.power && to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u)) > 0) ++over;
+        if (d + 1 == A.power && to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u)) > 0) {
+            if (!over) gd_log_element(A.status, A.tag, (unsigned long long)off[2]);
+            ++over;
+        }
+    }
+    if (over) atomicAdd(&A.status->gdecomp_overflow, over);
+}
+
+// Contiguous a and result (digit axis innermost): a workgroup takes 256
+// consecutive elements, one per work-item, runs the reference's digit loop
+// into LDS and stores the 256 * power digits as one contiguous run, so that
+// both the loads and the stores are whole lines and every element is read once.
+constexpr int kGdFlatMaxPower = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gdecomp_flat(GdArgs A, int64_t n) {
+    __shared__ T buf[256 * kGdFlatMaxPower];
+    const uint64_t mask = (A.base_bits == 0) ? 0 : ((~0ull) >> (64 - A.base_bits));
+    const int P = (int)A.power;
+    const T* a = (const T*)A.a;
+    T* out = (T*)A.out;
+    unsigned long long over = 0;
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < n; base += (int64_t)gridDim.x * 256) {
+        const int cnt = (int)min<int64_t>(256, n - base);
+        if ((int)threadIdx.x < cnt) {
+            int64_t v = a[base + threadIdx.x];
+            for (int d = 0; d < P; ++d) {
+                const uint64_t u = (uint64_t)v;  // size_t arithmetic of the reference
+                buf[threadIdx.x * P + d] = (T)to_t<T>((int64_t)(u & mask));
+                v = to_t<T>((int64_t)(A.base_bits ? (u >> A.base_bits) : u));
+            }
+            if (v > 0) {
+                if (!over) gd_log_element(A.status, A.tag, (unsigned long long)(base + threadIdx.x));
+                ++over;
+            }
+        }
+        __syncthreads();
+        T* dst = out + base * P;
+        for (int i = threadIdx.x; i < cnt * P; i += 256) dst[i] = buf[i];
+        __syncthreads();
     }
     if (over) atomicAdd(&A.status->gdecomp_overflow, over);
 }
 
 // g_decomposition (example_impl/g_decomposition_impl.cpp:10-67)
-void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hipStream_t st) {
+void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits,
+                     unsigned long long tag, hipStream_t st) {
     check_view(a, "a");
     check_view(result, "result");
     require_int(result, result, "result");
@@ -783,13 +827,16 @@ void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, ui
     if (base_bits >= 64) raise(HX_ERR_INVALID_ARGUMENT, "base_bits must be < 64");
     if (power > 0xffffffffull) raise(HX_ERR_INVALID_ARGUMENT, "power too large");
     int64_t strides[kMaxOps][HX_MAX_DIMS] = {};
-    for (int i = 0; i < a->ndim; ++i) {
+    int64_t rm = 1;  // row-major strides of a's shape: the element's index for the overflow log
+    for (int i = a->ndim - 1; i >= 0; --i) {
         strides[0][i] = result->strides[i];
         strides[1][i] = a->strides[i];
+        strides[2][i] = rm;
+        rm *= a->dims[i];
     }
     GdArgs A;
     std::memset(&A, 0, sizeof A);
-    A.it = make_iter(a->ndim, a->dims, 2, strides, false);
+    A.it = make_iter(a->ndim, a->dims, 3, strides, false);
     if (A.it.total == 0) return;
     A.out = result->data;
     A.a = a->data;
@@ -797,8 +844,16 @@ void g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, ui
     A.power = (uint32_t)power;
     A.base_bits = (uint32_t)base_bits;
     A.status = device_status();
+    A.tag = tag;
     const int64_t nout = A.it.total * (int64_t)power;
-    if (A.digit_stride == 1 && power >= 1 && nout < (int64_t(1) << 31)) {
+    const bool flat = A.it.ndim == 1 && A.it.strides[1][0] == 1 && A.it.strides[2][0] == 1 &&
+                      A.it.strides[0][0] == (int64_t)power;
+    if (flat && A.digit_stride == 1 && power >= 1 && power <= (uint64_t)kGdFlatMaxPower) {
+        const unsigned g = grid_for(A.it.total, 256, 16384);
+        if (result->dtype == HX_INT32)
+            hipLaunchKernelGGL(k_gdecomp_flat<int32_t>, dim3(g), dim3(256), 0, st, A, A.it.total);
+        else hipLaunchKernelGGL(k_gdecomp_flat<int64_t>, dim3(g), dim3(256), 0, st, A, A.it.total);
+    } else if (A.digit_stride == 1 && power >= 1 && nout < (int64_t(1) << 31)) {
         const FastDiv pw = make_fastdiv((uint32_t)power);
         unsigned g = grid_for(nout, kBlock);
         if (result->dtype == HX_INT32) hipLaunchKernelGGL(k_gdecomp_digits<int32_t>, dim3(g), dim3(kBlock), 0, st, A, pw);
@@ -1171,7 +1226,14 @@ int hx_axis_modsum(const hx_view* a, const hx_view* p, const hx_view* result, in
 
 int hx_g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hx_stream stream) {
     HX_API_BEGIN
-    hx::g_decomposition(a, result, power, base_bits, as_stream(stream));
+    hx::g_decomposition(a, result, power, base_bits, 0, as_stream(stream));
+    HX_API_END
+}
+
+int hx_g_decomposition_tagged(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits,
+                              uint64_t tag, hx_stream stream) {
+    HX_API_BEGIN
+    hx::g_decomposition(a, result, power, base_bits, tag, as_stream(stream));
     HX_API_END
 }
 

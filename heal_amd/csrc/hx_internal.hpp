@@ -127,12 +127,26 @@ __device__ __forceinline__ void iter_offsets(const IterSpace& it, int64_t idx, i
 }
 
 // ------------------------------------------------------- device status
+constexpr int kGdLog = 64;  // = HX_GDECOMP_LOG_MAX
 struct DevStatus {
     unsigned long long gdecomp_overflow;
     unsigned long long index_out_of_range;
-    unsigned long long spare[6];
+    unsigned long long gd_log_count;  // entries claimed in gd_log (may pass kGdLog; only kGdLog are kept)
+    unsigned long long spare[5];
+    unsigned long long gd_log[2 * kGdLog];  // (tag, row-major index in a) of overflowing elements
 };
 DevStatus* device_status();  // current device's block (allocated on first use)
+
+// Records one g_decomposition element whose value was not exhausted; a
+// work-item calls it at most once, and not at all once the log is full.
+__device__ __forceinline__ void gd_log_element(DevStatus* st, unsigned long long tag, unsigned long long index) {
+    if (*(volatile unsigned long long*)&st->gd_log_count >= (unsigned long long)kGdLog) return;
+    const unsigned long long k = atomicAdd(&st->gd_log_count, 1ull);
+    if (k < (unsigned long long)kGdLog) {
+        st->gd_log[2 * k] = tag;
+        st->gd_log[2 * k + 1] = index;
+    }
+}
 
 // grid size for grid-stride kernels
 inline unsigned grid_for(int64_t total, int block, int64_t cap = 4096) {

@@ -409,18 +409,20 @@ int hx_abi_version(void) { return HX_ABI_VERSION; }
 
 const char* hx_last_error(void) { return t_err.c_str(); }
 
-int hx_status_poll(hx_stream stream, hx_device_status* out) {
+int hx_status_poll_log(hx_stream stream, hx_device_status* out, hx_gdecomp_log* log) {
     HX_API_BEGIN
-    // in the stream's order: counters are taken (read and cleared atomically)
-    // after this stream's earlier work; counts of work still running on other
-    // streams stay for the next poll
+    // in the stream's order: counters and log are taken (read and cleared
+    // atomically, word by word) after this stream's earlier work; counts of
+    // work still running on other streams stay for the next poll
+    static_assert(kGdLog == HX_GDECOMP_LOG_MAX, "log size");
     DevStatus* st = device_status();
     hipStream_t s = as_stream(stream);
     const int n = (int)(sizeof(DevStatus) / sizeof(unsigned long long));
+    static_assert(sizeof(DevStatus) / sizeof(unsigned long long) <= 256, "one workgroup takes the status");
     void* tmp = nullptr;
     check_hip(hipMallocAsync(&tmp, sizeof(DevStatus), s), "hipMallocAsync(status)");
     DevStatus h;
-    hipLaunchKernelGGL(k_status_take, dim3(1), dim3(64), 0, s, (unsigned long long*)st, (unsigned long long*)tmp, n);
+    hipLaunchKernelGGL(k_status_take, dim3(1), dim3(256), 0, s, (unsigned long long*)st, (unsigned long long*)tmp, n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&h, tmp, sizeof h, hipMemcpyDeviceToHost, s);
     (void)hipFreeAsync(tmp, s);
@@ -430,8 +432,18 @@ int hx_status_poll(hx_stream stream, hx_device_status* out) {
         out->gdecomp_overflow = h.gdecomp_overflow;
         out->index_out_of_range = h.index_out_of_range;
     }
+    if (log) {
+        const uint64_t k = std::min<uint64_t>(h.gd_log_count, (uint64_t)kGdLog);
+        log->count = k;
+        for (uint64_t i = 0; i < k; ++i) {
+            log->tag[i] = h.gd_log[2 * i];
+            log->index[i] = h.gd_log[2 * i + 1];
+        }
+    }
     HX_API_END
 }
+
+int hx_status_poll(hx_stream stream, hx_device_status* out) { return hx_status_poll_log(stream, out, nullptr); }
 
 int hx_malloc(void** out, size_t bytes, hx_stream stream) {
     HX_API_BEGIN

@@ -78,6 +78,24 @@ const char* hx_last_error(void);
  * (may be NULL) and synchronises `stream` only. */
 int hx_status_poll(hx_stream stream, hx_device_status* out);
 
+/* The elements behind gdecomp_overflow, for the reference's per-element
+ * warning "Warning: value at <coords> exceeds capacity with base_bits=B and
+ * power=P" (example_impl/g_decomposition_impl.cpp:58-65): up to
+ * HX_GDECOMP_LOG_MAX entries (the call's tag from hx_g_decomposition_tagged,
+ * 0 for hx_g_decomposition, and the element's row-major index in `a`), in no
+ * particular order; further elements are counted in gdecomp_overflow only.
+ * An entry written by work that runs on another stream while the poll takes
+ * the log can be lost. */
+#define HX_GDECOMP_LOG_MAX 64
+typedef struct hx_gdecomp_log {
+    uint64_t count;                      /* entries filled */
+    uint64_t tag[HX_GDECOMP_LOG_MAX];
+    uint64_t index[HX_GDECOMP_LOG_MAX];
+} hx_gdecomp_log;
+
+/* hx_status_poll that also takes the g_decomposition overflow log (log may be NULL). */
+int hx_status_poll_log(hx_stream stream, hx_device_status* out, hx_gdecomp_log* log);
+
 /* ---------------------------------------------------- memory (L0 of the API)
  * Replaces the storage side of DeviceTensor / allocate_on_hardware /
  * host_to_device / device_to_host
@@ -163,6 +181,11 @@ int hx_set_const(const hx_view* a, int64_t ivalue, double fvalue, hx_stream stre
  * Elements whose value is not exhausted after `power` digits increment the
  * gdecomp_overflow counter (the reference prints a warning). */
 int hx_g_decomposition(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits, hx_stream stream);
+/* hx_g_decomposition whose overflowing elements carry `tag` in the status log
+ * (hx_status_poll_log), so that the caller can print the reference's
+ * per-element warning with the call's shape, base_bits and power. */
+int hx_g_decomposition_tagged(const hx_view* a, const hx_view* result, uint64_t power, uint64_t base_bits,
+                              uint64_t tag, hx_stream stream);
 
 /* Negacyclic NTT / INTT over axis 1 of [l, m, r, k] views
  * (reference include/ntt.h:31-52, example_impl/ntt_impl.cpp:69-170).

@@ -143,13 +143,33 @@ def test_permute_cache_identical_rows(lhw):
 
 
 def test_gdecomp_overflow_warning_reported_on_sync(lhw, capfd):
+    """The reference's per-element warning (example_impl/g_decomposition_impl.cpp:58-65),
+    printed at the next synchronisation."""
     a = lhw.host_to_device_64(torch.tensor([5, 1 << 40, 7], dtype=torch.int64))
     r = lhw.allocate_on_hardware_64([3, 2])
     lhw.g_decomposition_64(a, r, 2, 8)
     out = lhw.device_to_host_64(r)
     assert out.tolist() == [[5, 0], [0, 0], [7, 0]]
     err = capfd.readouterr().err
-    assert "exceed capacity" in err and "base_bits=8" in err
+    assert err == "Warning: value at 1 exceeds capacity with base_bits=8 and power=2\n"
+    # two calls of other shapes before one synchronisation: each element with its call's coordinates
+    x = torch.zeros(2, 3, dtype=torch.int64)
+    x[1, 2] = 1 << 20
+    x[0, 1] = -1  # size_t digit arithmetic: -1 leaves 2^56 - 1 after two 4-bit digits
+    b = lhw.host_to_device_64(x)
+    rb = lhw.allocate_on_hardware_64([2, 3, 2])
+    lhw.g_decomposition_64(b, rb, 2, 4)
+    c = lhw.host_to_device_64(torch.tensor([[1 << 30, 1], [3, 1 << 30]], dtype=torch.int64))
+    rc = lhw.allocate_on_hardware_64([2, 2, 3])
+    lhw.g_decomposition_64(c, rc, 3, 5)
+    lhw.device_to_host_64(rc)
+    err = capfd.readouterr().err.splitlines()
+    assert err == ["Warning: value at 0 1 exceeds capacity with base_bits=4 and power=2",
+                   "Warning: value at 1 2 exceeds capacity with base_bits=4 and power=2",
+                   "Warning: value at 0 0 exceeds capacity with base_bits=5 and power=3",
+                   "Warning: value at 1 1 exceeds capacity with base_bits=5 and power=3"], err
+    lhw.device_to_host_64(rc)
+    assert capfd.readouterr().err == ""
 
 
 def test_allocate_on_hardware_returns_zeros(lhw):
