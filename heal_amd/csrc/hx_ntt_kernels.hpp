@@ -839,7 +839,8 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
                      (int64_t)r16_wl<H>(L_OUT, lane) * wstride;
         if (es == 8) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) *(uint64_t*)(base + r16_wc<H>(L_OUT, e) * wstride) = x[e];
+            for (int e = 0; e < 16; ++e)  // streaming: whole lines, read next by another launch
+                __builtin_nontemporal_store(x[e], (uint64_t*)(base + r16_wc<H>(L_OUT, e) * wstride));
         } else {
 #pragma unroll
             for (int e = 0; e < 16; ++e) *(int32_t*)(base + r16_wc<H>(L_OUT, e) * wstride) = (int32_t)x[e];
