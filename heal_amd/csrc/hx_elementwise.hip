@@ -162,7 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_modop_rows(ElemArgs A, FastDiv segs_
         longlong2 r;
         r.x = (int64_t)modop_fast(OP, table[t], av.x, bv.x);
         r.y = (int64_t)modop_fast(OP, table[t + 1], av.y, bv.y);
-        *(longlong2*)((int64_t*)A.out + o0 + t) = r;
+        __builtin_nontemporal_store(r.x, (int64_t*)A.out + o0 + t);
+        __builtin_nontemporal_store(r.y, (int64_t*)A.out + o0 + t + 1);
     }
 }
 
@@ -808,7 +809,7 @@ __global__ __launch_bounds__(256) void k_gdecomp_flat(GdArgs A, int64_t n) {
         }
         __syncthreads();
         T* dst = out + base * P;
-        for (int i = threadIdx.x; i < cnt * P; i += 256) dst[i] = buf[i];
+        for (int i = threadIdx.x; i < cnt * P; i += 256) __builtin_nontemporal_store(buf[i], dst + i);
         __syncthreads();
     }
     if (over) atomicAdd(&A.status->gdecomp_overflow, over);
@@ -941,7 +942,8 @@ __global__ __launch_bounds__(kBlock) void k_permute_rows(PermArgs A, FastDiv seg
         longlong2 v = make_longlong2(0, 0);
         if (ok) v = *(const longlong2*)((const int64_t*)A.a + src + t);
         else bad += 2;
-        *(longlong2*)((int64_t*)A.out + ooff + t) = v;
+        __builtin_nontemporal_store(v.x, (int64_t*)A.out + ooff + t);
+        __builtin_nontemporal_store(v.y, (int64_t*)A.out + ooff + t + 1);
     }
     if (bad) atomicAdd(&A.status->index_out_of_range, bad);
 }
