@@ -64,6 +64,7 @@ struct InnerArgs {
     int64_t n_tg, n_hi;
     int32_t H;            // the inner window (= logm - hhi)
     int32_t tpt, per;     // grouped launches: threads per tile, tiles per workgroup
+    FastDiv fd_B, fd_ntg; // 32-bit decoding of the tile index (divisors B and n_tg)
 };
 
 // LDS rows of K2 (64-bit slots): one pad slot every 16 and one more per row,
@@ -186,11 +187,12 @@ __device__ __forceinline__ void ks_tile(const InnerArgs& A, int64_t blk, int64_t
     // instances vary fastest, then limb groups: the workgroups of one XCD share
     // their twiddles (same limbs and rows) and the partial cache lines at the
     // edges of neighbouring column groups of the same rows, in one L2
-    int64_t tile = xcd_tile(blk, nblk);
-    const int64_t b = tile % A.B;
-    tile /= A.B;
-    const int64_t tg = tile % A.n_tg;
-    const int64_t hi = tile / A.n_tg;
+    const uint32_t tile = (uint32_t)xcd_tile(blk, nblk);
+    const uint32_t tb = fdiv(tile, A.fd_B);
+    const int64_t b = tile - tb * A.fd_B.d;
+    const uint32_t thi = fdiv(tb, A.fd_ntg);
+    const int64_t tg = tb - thi * A.fd_ntg.d;
+    const int64_t hi = thi;
     const int64_t t0 = tg * G;
     const int64_t ncols1 = (int64_t)D * A.k * A.B, ncols2 = 2 * A.k * A.B;
     const int64_t row0 = hi * W;
@@ -758,6 +760,9 @@ bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void
     if (K.threads2 > 512 || K.lds2 > 64 * 1024)
         raise(HX_ERR_INVALID_ARGUMENT, "dnum too large for the fused key-switch tile");
     K.blocks2 = A.n_hi * A.n_tg * B;
+    if (K.blocks2 > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
+    A.fd_B = make_fastdiv((uint32_t)B);
+    A.fd_ntg = make_fastdiv((uint32_t)A.n_tg);
 
     // K3: inverse pass over the high bits, m^-1, Galois scatter into out
     PassArgs& P3 = K.P3;

@@ -215,6 +215,11 @@ struct PassArgs {
     int32_t S_cols, S_bot;   // tile = S_cols columns x S_bot "bot" values
     int64_t bot_count;       // 2^b0
     int64_t n_bot_tiles, n_col_tiles;
+    // 32-bit decoding of tile and column indices (set by pass_blocks, which
+    // bounds the tile count and ncols below 2^31): divisors n_col_tiles,
+    // n_bot_tiles, cdim[0], cdim[1]; S_cols = 2^log_scols
+    FastDiv fd_ctiles, fd_btiles, fd_c0, fd_c1;
+    int32_t log_scols;
     int32_t reduce_in;
     int32_t scale_out;
     int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 8q) forward, [0, 4q) inverse)
@@ -689,9 +694,54 @@ struct R16Lds {
     __device__ static constexpr int slot(int w) { return LM ? w + (w >> 4) : w; }
 };
 
+// Sub-problem s of tile `blk` of a radix-16 pass, decoded by each of its
+// threads on its own (no barrier before the loads): column, window base row,
+// offsets, limb and digit, and whether an earlier pass flagged the column.
+struct R16Sub {
+    int64_t src_off, dst_off, alt_off, jbase, col, top;
+    int32_t limb, digit, valid, skip;
+};
+
+// Sub-problem s of tile `tile` (already mapped to the tile order; in_range
+// false gives a sub-problem that is not valid).
+__device__ __forceinline__ R16Sub r16_sub_at(const PassArgs& P, uint32_t tile, bool in_range, int s) {
+    R16Sub u;
+    const uint32_t tq = fdiv(tile, P.fd_ctiles);
+    const uint32_t col_tile = tile - tq * P.fd_ctiles.d;
+    const uint32_t top = fdiv(tq, P.fd_btiles);
+    const uint32_t bot_tile = tq - top * P.fd_btiles.d;
+    const uint32_t cs = (uint32_t)s & ((1u << P.log_scols) - 1u), bs = (uint32_t)s >> P.log_scols;
+    const int64_t col = ((int64_t)col_tile << P.log_scols) + cs;
+    const int64_t bot = (int64_t)bot_tile * P.S_bot + bs;
+    const bool valid = in_range && col < P.ncols && bot < P.bot_count;
+    const uint32_t c = valid ? (uint32_t)col : 0u;
+    const uint32_t c1 = fdiv(c, P.fd_c0);
+    const uint32_t c2 = fdiv(c1, P.fd_c1);
+    const int64_t coord[3] = {(int64_t)(c - c1 * P.fd_c0.d), (int64_t)(c1 - c2 * P.fd_c1.d), (int64_t)c2};
+    // the intermediate buffer addresses columns by their linear index
+    const int64_t lcoord[3] = {(int64_t)c, 0, 0};
+    u.src_off = P.src.kind == 2 ? col_offset(P.src, lcoord) : col_offset(P.src, coord);
+    u.dst_off = P.dst.kind == 2 ? col_offset(P.dst, lcoord) : col_offset(P.dst, coord);
+    u.alt_off = P.alt.ptr ? col_offset(P.alt, coord) : 0;
+    u.limb = (int32_t)coord[P.limb_axis];
+    u.digit = P.src.kind == 3 ? (int32_t)coord[P.gd_axis] : 0;
+    u.valid = valid;
+    u.top = top;
+    u.jbase = ((int64_t)top << (P.b0 + P.H)) + (valid ? bot : 0);
+    u.col = c;
+    u.skip = (P.colflag && P.flag_skip && valid) ? (int32_t)P.colflag[c] : 0;
+    return u;
+}
+
+__device__ __forceinline__ R16Sub r16_sub(const PassArgs& P, int64_t blk, int64_t nblk, int s) {
+    const bool in_range = blk < nblk;
+    return r16_sub_at(P, in_range ? (uint32_t)xcd_tile(blk, nblk) : 0u, in_range, s);
+}
+
 // Tile `blk` of a radix-16 pass's `nblk` tiles, served by the 32 * 2^(H-4)
-// threads tid = 0.. of a workgroup with tile bookkeeping T and an LDS block xs
-// of [32][SP] slots.  Every thread of the workgroup runs the same barriers.
+// threads tid = 0.. of a workgroup with tile bookkeeping T (used only when one
+// pass both flags and skips columns) and an LDS block xs of [32][SP] slots.
+// Every thread of the workgroup runs the same barriers.
 template <int H, bool INV, bool LM = false>
 __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t nblk, int tid,
                                          TileSharedN<32>& T, uint64_t* xs) {
@@ -700,29 +750,36 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
     constexpr int SP = R16Lds<H, LM>::SP;
     constexpr int L_IN = INV ? 1 : 0;   // layout of the first stages
     constexpr int L_OUT = 1 - L_IN;
-    const int64_t top = tile_setup(P, S, T, blk, nblk, tid);
-
     constexpr int TL = W / 16;
     const int s = LM ? tid / TL : tid % S;
     const int lane = LM ? tid % TL : tid / S;
-    const bool valid = T.valid[s];
-    const int limb = T.limb[s];
+    const R16Sub u = r16_sub(P, blk, nblk, s);
+    const int64_t top = u.top;
+    // a pass that flags columns and also withholds their results (a single-pass
+    // transform) combines the flags of a sub-problem's threads in T.bad,
+    // cleared here and set after the exchange barrier
+    const bool flag_here = P.colflag && P.flag_set && P.flag_skip;
+    if (flag_here && lane == 0) T.bad[s] = 0;
+    bool mybad = false;
+
+    const bool valid = u.valid;
+    const int limb = u.limb;
     const LimbInfo& L = P.limbs[limb];
     const uint64_t q = L.q;
     const int mode = limb_mode(L);
-    const int64_t jbase = T.jbase[s];
+    const int64_t jbase = u.jbase;
     uint64_t x[16];
 
     // ---- load straight into the first layout
     if (valid) {
         const int es = P.src.kind == 1 ? 4 : 8;
         const int64_t wstride = (int64_t)es * (P.src.row_stride << P.b0);  // bytes per window row
-        const char* base = (const char*)P.src.ptr + es * (T.src_off[s] + jbase * P.src.row_stride) +
+        const char* base = (const char*)P.src.ptr + es * (u.src_off + jbase * P.src.row_stride) +
                            (int64_t)r16_wl<H>(L_IN, lane) * wstride;
         if (P.src.kind == 3) {
             // digit d of x: the reference's unsigned digit loop
             // (example_impl/g_decomposition_impl.cpp:52-56) in closed form
-            const int d = T.digit[s];
+            const int d = u.digit;
             const int wb = P.gd_w;
             const uint64_t mask = wb ? (~0ull >> (64 - wb)) : 0;
             const int shift = wb * d;
@@ -776,7 +833,7 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
                         r[e + 3] = v4.w;
                     }
                 }
-                const char* colp = (const char*)P.src.ptr + 8 * T.src_off[s];
+                const char* colp = (const char*)P.src.ptr + 8 * u.src_off;
                 const int64_t rs = 8 * P.src.row_stride;
 #pragma unroll
                 for (int e = 0; e < 16; ++e) xv[e] = *(const int64_t*)(colp + (int64_t)r[e] * rs);
@@ -785,8 +842,8 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
                 for (int e = 0; e < 16; ++e) {
                     const int64_t j = jbase + ((int64_t)r16_w<H>(L_IN, e, lane) << P.b0);
                     const int32_t r = P.src_map[j];
-                    xv[e] = r >= 0 ? load_any(P.src, T.src_off[s] + (int64_t)r * P.src.row_stride)
-                                   : load_any(P.alt, T.alt_off[s] + j * P.alt.row_stride);
+                    xv[e] = r >= 0 ? load_any(P.src, u.src_off + (int64_t)r * P.src.row_stride)
+                                   : load_any(P.alt, u.alt_off + j * P.alt.row_stride);
                 }
             }
 #pragma unroll
@@ -800,7 +857,10 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
                 bool bad = false;
 #pragma unroll
                 for (int e = 0; e < 16; ++e) bad |= (uint64_t)xv[e] > lim;
-                if (bad) mark_flagged(P, T, s);
+                if (bad) {
+                    mybad = true;
+                    P.colflag[u.col] = 1u;
+                }
             }
         }
     } else {
@@ -817,6 +877,12 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) x[e] = xs_s[R16Lds<H, LM>::slot(r16_w<H>(L_OUT, e, lane))];
+    bool withhold = u.skip != 0;
+    if (flag_here) {  // uniform over the workgroup
+        if (mybad) T.bad[s] = 1;
+        __syncthreads();
+        withhold = withhold || T.bad[s];
+    }
     if (!valid) return;
     // the last pass of an inverse transform ends with the global stage logm - 1
     // (window bit H - 1, the last stage of layout A), where m^-1 is folded in
@@ -828,11 +894,11 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
         for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], q);
     }
     // a flagged column is written by the reference-semantics kernel instead
-    if (P.flag_skip && (T.skip[s] || T.bad[s])) return;
+    if (P.flag_skip && withhold) return;
 
     // ---- store from registers
     const int es = P.dst.kind == 1 ? 4 : 8;
-    const int64_t colbase = T.dst_off[s];
+    const int64_t colbase = u.dst_off;
     if (!P.dst_map && !P.dst_map_limb) {
         const int64_t wstride = (int64_t)es * (P.dst.row_stride << P.b0);
         char* base = (char*)P.dst.ptr + es * (colbase + jbase * P.dst.row_stride) +
@@ -870,6 +936,7 @@ template <int H, bool INV, bool LM>
 __global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
     r16_pass<H, INV, LM>(P, blockIdx.x, gridDim.x);
 }
+
 
 // The radix-16 pass over the low index bits (b0 = 0) of one rows-contiguous
 // column whose row map is the bit reversal of log m bits: the forward pass
@@ -1264,7 +1331,13 @@ int64_t pass_blocks(PassArgs& P) {
     P.n_bot_tiles = (P.bot_count + P.S_bot - 1) / P.S_bot;
     P.n_col_tiles = (P.ncols + P.S_cols - 1) / P.S_cols;
     const int64_t blocks = ntop * P.n_bot_tiles * P.n_col_tiles;
-    if (blocks > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
+    if (blocks > 0x7fffffff || P.ncols > 0x7fffffff) raise(HX_ERR_INVALID_ARGUMENT, "NTT too large for one launch");
+    P.fd_ctiles = make_fastdiv((uint32_t)std::max<int64_t>(1, P.n_col_tiles));
+    P.fd_btiles = make_fastdiv((uint32_t)std::max<int64_t>(1, P.n_bot_tiles));
+    P.fd_c0 = make_fastdiv((uint32_t)std::max<int64_t>(1, P.cdim[0]));
+    P.fd_c1 = make_fastdiv((uint32_t)std::max<int64_t>(1, P.cdim[1]));
+    P.log_scols = 0;
+    while ((1 << P.log_scols) < P.S_cols) ++P.log_scols;
     return blocks;
 }
 
