@@ -26,15 +26,15 @@ SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs at 2.4 GHz
 # Profiled constants of the current kernels on the bench workload, keyed by
 # (N, L, B); each entry names the profile it comes from and goes stale when the
 # kernel changes.  HBM bytes per launch: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, each scaled by the same counter's reading for a 256 MiB copy.
-# VALU instructions per launch: SQ_INSTS_VALU (each issues in one quad-cycle,
-# SQ_ACTIVE_INST_VALU = SQ_INSTS_VALU within 0.2 %).
+# WRITE_SIZE (one pass each), scaled by the same counter's reading for a
+# 256 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1).  VALU instructions per
+# launch: SQ_INSTS_VALU.  The NTT pass figures average its two launches.
 PROFILED = {
     (65536, 24, 16): {
-        "ntt_pass_traffic": (1305184556, "profiles/ntt_pass_hbm_traffic.md"),
-        "ntt_pass_valu": (133886976, "profiles/ntt_pass_sq_counters.md"),
-        "k2_traffic": (2437285478, "profiles/ntt_pass_hbm_traffic.md"),
-        "k2_valu": (259178496, "profiles/ntt_pass_sq_counters.md"),
+        "ntt_pass_traffic": (1278601592, "profiles/ntt_pass_hbm_traffic.md"),
+        "ntt_pass_valu": (137170944, "profiles/ntt_pass_sq_counters.md"),
+        "k2_traffic": (2427128704, "profiles/ntt_pass_hbm_traffic.md"),
+        "k2_valu": (258318336, "profiles/ntt_pass_sq_counters.md"),
     }
 }
 
@@ -250,12 +250,27 @@ def main():
     k2_traffic, k2_traffic_src = profiled("k2_traffic")
     ntt_valu, ntt_valu_src = profiled("ntt_pass_valu")
 
-    def valu_entry(insts, src, seconds):
-        if insts is None:
-            return None
-        issue = insts * 4 / SIMDS / CLOCK_HZ  # quad-cycles on 1024 SIMDs
-        return {"insts_per_launch": insts, "issue_ms_at_one_quad_cycle": round(issue * 1e3, 4),
-                "frac": round(issue / seconds, 3), "insts_source": src}
+    # VALU ceiling: the launch's radix-2 butterflies at the rate the GPU runs the
+    # passes' own 60-bit lazy butterfly with its operands in registers
+    # (hx_butterfly_rate, measured here); frac = that time / the launch time
+    bfly_rate = capi.butterfly_rate()
+
+    def valu_entry(insts, src, seconds, butterflies):
+        e = {"butterflies_per_launch": butterflies, "butterfly_rate_G_per_s": round(bfly_rate / 1e9, 1),
+             "ceiling_ms": round(butterflies / bfly_rate * 1e3, 4),
+             "frac": round(butterflies / bfly_rate / seconds, 3)}
+        if insts is not None:
+            issue = insts * 4 / SIMDS / CLOCK_HZ  # quad-cycles on 1024 SIMDs
+            e.update({"insts_per_launch": insts, "issue_ms_at_one_quad_cycle": round(issue * 1e3, 4),
+                      "insts_source": src})
+        return e
+
+    # butterflies: a launch of the NTT pass runs log2(N)/2 stages of N/2 per limb;
+    # K2 runs the inner window (8 stages of N/2) forward on D and inverse on 2
+    # columns per limb of every instance
+    logn = N.bit_length() - 1
+    ntt_bflies = limbs_fwd * (N // 2) * (logn // n_launch)
+    k2_bflies = B * L * (D + 2) * (N // 2) * batched.fused.config()["low_bits"]
 
     result = {
         "metric": "key-switches/sec (N=2^16, L=24, dnum=3, w=60), batched fused form; its hipGraph replay, the "
@@ -293,7 +308,7 @@ def main():
                      "bytes_per_launch": k2_bytes, "avg_launch_ms": round(k_ms[1], 4),
                      "kernel_ms": {"K1": round(k_ms[0], 4), "K2": round(k_ms[1], 4), "K3": round(k_ms[2], 4)},
                      "marked_step_ms": round(t_marked / args.steps * 1e3, 4),
-                     "valu": valu_entry(k2_valu, k2_valu_src, k2_s),
+                     "valu": valu_entry(k2_valu, k2_valu_src, k2_s, k2_bflies),
                      "keyswitch_GBps": round(batched.algorithmic_bytes() * args.steps / t_fused / 1e9, 1),
                      "keyswitch_frac": round(batched.algorithmic_bytes() * args.steps / t_fused / 1e9 / HBM_PEAK_GBS,
                                              4)},
@@ -305,7 +320,7 @@ def main():
                               "traffic_source": profiled("ntt_pass_traffic")[1],
                               "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4),
                               "launch_read_write_GBps": round(2 * achieved, 1),
-                              "valu": valu_entry(ntt_valu, ntt_valu_src, avg_launch)},
+                              "valu": valu_entry(ntt_valu, ntt_valu_src, avg_launch, ntt_bflies)},
         "cpu_baseline": None,
     }
     try:

@@ -99,6 +99,7 @@ _PROTOS = {
     "hx_ntt_plan_workspace_bytes": [ctypes.c_void_p, VP, VP, ctypes.POINTER(ctypes.c_size_t)],
     "hx_ntt_plan_exec_ws": [ctypes.c_void_p, VP, VP, ctypes.c_void_p, ctypes.c_size_t, STREAM],
     "hx_ntt_plan_exec_noalloc": [ctypes.c_void_p, VP, VP, STREAM],
+    "hx_butterfly_rate": [ctypes.POINTER(ctypes.c_double), STREAM],
     "hx_keyswitch_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p, VP, U64, U64,
                                  STREAM],
     "hx_keyswitch_key_bytes": [ctypes.c_void_p, I64, ctypes.POINTER(ctypes.c_size_t)],
@@ -359,6 +360,14 @@ def stream_capturing(stream: int | None = None) -> bool:
     c = ctypes.c_int(0)
     check(lib.hx_stream_capturing(stream if stream is not None else stream_of(), ctypes.byref(c)))
     return bool(c.value)
+
+
+def butterfly_rate(stream: int | None = None) -> float:
+    """Butterflies per second of the NTT passes' 60-bit lazy butterfly with its
+    operands in registers (the arithmetic ceiling of a pass on this GPU)."""
+    out = ctypes.c_double(0)
+    check(lib.hx_butterfly_rate(ctypes.byref(out), stream if stream is not None else stream_of()))
+    return out.value
 
 
 def status_poll(stream: int | None = None) -> dict:

@@ -462,11 +462,84 @@ void ntt_plan_exec_noalloc(const hx_ntt_plan* plan, const hx_view* a, const hx_v
     ntt_plan_run(plan, a, result, E, (uint32_t*)ptr, ptr + E.flag_bytes, st);
 }
 
+// ------------------------------------------------------------ VALU ceiling
+// Throughput of the passes' 60-bit butterfly (MODE_L64, forward) with its
+// operands in registers: every thread runs 16 values as 8 independent
+// butterflies per round, each with its own twiddle pair, at 8 waves per SIMD.  No memory traffic but one store per thread at the end, so its rate
+// is the arithmetic ceiling of a pass of these butterflies on this GPU.
+constexpr int kRateRounds = 1024;
+
+__global__ __launch_bounds__(256) void k_butterfly_rate(uint64_t* out, uint64_t q, uint64_t seed) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = ((t + 1) * 0x9e3779b97f4a7c15ull + (uint64_t)i * seed) % q;
+    uint64_t w[8], wp[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        w[i] = ((seed ^ t) + (uint64_t)i * 0x1234567ull) % q;
+        wp[i] = (w[i] * 0x2545f4914f6cdd1dull) | 1u;
+    }
+    for (int r = 0; r < kRateRounds; ++r) {
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) butterfly<MODE_L64, false>(x[i], x[i + 1], w[i / 2], wp[i / 2], q);
+    }
+    uint64_t h = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) h ^= x[i];
+    out[t] = h;
+}
+
+// Butterflies per second of k_butterfly_rate over the whole GPU (best of
+// three launches of 8 workgroups of 4 waves per CU).
+double butterfly_rate(hipStream_t st) {
+    int dev = 0, cus = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    const int blocks = std::max(1, cus) * 8, threads = 256;
+    uint64_t* out = nullptr;
+    check_hip(hipMallocAsync((void**)&out, (size_t)blocks * threads * 8, st), "hipMallocAsync(rate)");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best = 1e30f;
+    try {
+        check_hip(hipEventCreate(&e0), "hipEventCreate");
+        check_hip(hipEventCreate(&e1), "hipEventCreate");
+        const uint64_t q = 1152921504606844417ull;  // 2^60 - 2^15 + 1
+        for (int r = 0; r < 4; ++r) {
+            check_hip(hipEventRecord(e0, st), "hipEventRecord");
+            hipLaunchKernelGGL(k_butterfly_rate, dim3(blocks), dim3(threads), 0, st, out, q, (uint64_t)(r + 3));
+            check_hip(hipGetLastError(), "butterfly rate launch");
+            check_hip(hipEventRecord(e1, st), "hipEventRecord");
+            check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (r > 0) best = std::min(best, ms);  // the first launch warms up
+        }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFreeAsync(out, st);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_hip(hipFreeAsync(out, st), "hipFreeAsync(rate)");
+    const double bflies = (double)blocks * threads * kRateRounds * 8;
+    return bflies / (best * 1e-3);
+}
+
 }  // namespace hx
 
 using namespace hx;
 
 extern "C" {
+
+int hx_butterfly_rate(double* butterflies_per_s, hx_stream stream) {
+    HX_API_BEGIN
+    if (!butterflies_per_s) raise(HX_ERR_INVALID_ARGUMENT, "butterflies_per_s pointer must not be null.");
+    *butterflies_per_s = butterfly_rate(as_stream(stream));
+    HX_API_END
+}
 
 int hx_ntt_plan_create(hx_ntt_plan** out, int inverse, const hx_view* p, const hx_view* perm, const hx_view* twiddles,
                        const hx_view* m_inv, hx_stream stream) {

@@ -241,6 +241,11 @@ int hx_ntt_plan_exec_ws(const hx_ntt_plan* plan, const hx_view* a, const hx_view
  * threads) race and must be ordered by the caller. */
 #define HX_NTT_NOALLOC_SCRATCH_BYTES (64u << 20)
 int hx_ntt_plan_exec_noalloc(const hx_ntt_plan* plan, const hx_view* a, const hx_view* result, hx_stream stream);
+/* Measurement hook, no counterpart in the reference: the rate (butterflies per
+ * second, whole GPU) of the NTT passes' 60-bit lazy butterfly with operands in
+ * registers, i.e. the arithmetic ceiling of a pass of such butterflies on this
+ * GPU.  Synchronises the stream. */
+int hx_butterfly_rate(double* butterflies_per_s, hx_stream stream);
 /* permute without the host synchronisation: perms must have been validated
  * (e.g. by an earlier hx_permute); out-of-range entries still never read
  * outside `a`, they write 0 and count in index_out_of_range. */

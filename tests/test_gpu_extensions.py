@@ -211,3 +211,14 @@ def test_get_slice_views(lhw, engine):
         engine.get_slice(t, (4,))
     with pytest.raises(ValueError):
         engine.get_slice(t, (slice(None, None, -1),))
+
+
+def test_butterfly_rate_hook(capi):
+    """hx_butterfly_rate (the bench's VALU ceiling): a positive rate between a
+    tenth and ten times 1024 SIMDs x 2.4 GHz / (100 cycles per wave of 64
+    butterflies), repeatable within a factor of two."""
+    nominal = 1024 * 2.4e9 * 64 / 100
+    r1, r2 = capi.butterfly_rate(), capi.butterfly_rate()
+    for r in (r1, r2):
+        assert nominal / 10 < r < nominal * 10
+    assert 0.5 < r1 / r2 < 2.0
