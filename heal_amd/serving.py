@@ -40,12 +40,14 @@ Serving model:
 Latency runs from a request's arrival (Poisson arrivals, seeded) to the
 device end time of the step that completes it (events with timing, against
 an event recorded at the start of serve()).  serve() also returns a per-step
-timeline: pool, batch, rotation, host enqueue time and device start/end.
+timeline: pool, batch, rotation, host enqueue time and duration, and device
+start/end.
 """
 from __future__ import annotations
 
 import argparse
 import collections
+import gc
 import json
 import math
 import time
@@ -107,6 +109,7 @@ class _Step:
     start: torch.cuda.Event
     end: torch.cuda.Event
     finished: list = field(default_factory=list)
+    enqueue_s: float = 0.0  # host time spent enqueueing the step (its device interval includes it)
 
 
 class _Ring:
@@ -185,6 +188,7 @@ class _Pool:
         """Queues rotation `cursor` for every admitted request, then the retirement
         of the requests it completes."""
         B, r, ring = len(self.reqs), self.cursor, self.ring
+        h0 = time.perf_counter()
         rec = _Step(self.key, r, B, now, torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         with torch.cuda.stream(self.stream):
             rec.start.record()
@@ -195,6 +199,7 @@ class _Pool:
                                 row_map=ring.row_maps[r])
                 capi.rot_mac(self.acc[:B], self.out[:B], self.ct[:B], ring.perm_rows[r], self.diags[:B], r, self.q)
             rec.end.record()
+            rec.enqueue_s = time.perf_counter() - h0
             for q in self.reqs:
                 q.steps += 1
             rec.finished = self._retire(keep_results)
@@ -252,14 +257,16 @@ class ServingEngine:
 
     def warm(self, classes):
         """Creates the pools (plans, buffers) of the given (N, L) classes ahead of time
-        and runs one step of each kind on every pool's stream (first-use set-up:
-        stream workspaces, generators), then leaves the pools empty."""
+        and runs one step of each kind on every pool's stream with every slot
+        filled (first-use set-up: stream workspaces at their largest, generators,
+        allocator blocks), then leaves the pools empty."""
         for n, L in classes:
             self.pool(n, L)
         for p in self.pools.values():
             if p.reqs or p.inflight:
                 continue
-            p.admit(Request(rid=-1, n=p.n, limbs=p.L, seed=0, arrival=0.0))
+            for s in range(p.max_slots):
+                p.admit(Request(rid=-1 - s, n=p.n, limbs=p.L, seed=s, arrival=0.0))
             for _ in range(min(2, p.ring.R)):
                 p.step(0.0, False)
             p.reqs.clear()
@@ -282,6 +289,21 @@ class ServingEngine:
         done: list[Request] = []
         timeline = []
         torch.cuda.synchronize()
+        # the collector's pauses would land inside steps being enqueued (a step's
+        # device interval runs from its first to its last enqueued event): it is
+        # off while serving, and the objects alive now are left out of later passes
+        gc_was_on = gc.isenabled()
+        gc.collect()
+        gc.freeze()
+        gc.disable()
+        try:
+            return self._serve(pending, requests, keep_results, done, timeline)
+        finally:
+            if gc_was_on:
+                gc.enable()
+            gc.unfreeze()
+
+    def _serve(self, pending, requests, keep_results, done, timeline) -> dict:
         origin = torch.cuda.Event(enable_timing=True)
         origin.record()
         origin.synchronize()
@@ -318,8 +340,9 @@ class ServingEngine:
                         req.finished = end
                         done.append(req)
                     timeline.append({"pool": list(rec.pool), "r": rec.r, "batch": rec.batch,
-                                     "enqueued_s": round(rec.enqueued, 6), "start_s": round(start, 6),
-                                     "end_s": round(end, 6), "finished": [q.rid for q in rec.finished]})
+                                     "enqueued_s": round(rec.enqueued, 6), "enqueue_ms": round(rec.enqueue_s * 1e3, 3),
+                                     "start_s": round(start, 6), "end_s": round(end, 6),
+                                     "finished": [q.rid for q in rec.finished]})
                     progressed = True
             if not progressed:
                 waiting = any(p.inflight for p in self.pools.values())
