@@ -738,13 +738,22 @@ __device__ __forceinline__ R16Sub r16_sub(const PassArgs& P, int64_t blk, int64_
     return r16_sub_at(P, in_range ? (uint32_t)xcd_tile(blk, nblk) : 0u, in_range, s);
 }
 
+// Per-workgroup bookkeeping of a radix-16 tile: the combined flags of a
+// sub-problem's threads (a pass that both flags and withholds columns), and
+// the destination rows of the tile's window indices when the pass scatters
+// through a row map shared by all columns (loaded with the data, read at the
+// store).
+struct R16Shared {
+    int32_t bad[32];
+    int32_t rows[256];
+};
+
 // Tile `blk` of a radix-16 pass's `nblk` tiles, served by the 32 * 2^(H-4)
-// threads tid = 0.. of a workgroup with tile bookkeeping T (used only when one
-// pass both flags and skips columns) and an LDS block xs of [32][SP] slots.
-// Every thread of the workgroup runs the same barriers.
+// threads tid = 0.. of a workgroup with tile bookkeeping T and an LDS block xs
+// of [32][SP] slots.  Every thread of the workgroup runs the same barriers.
 template <int H, bool INV, bool LM = false>
-__device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t nblk, int tid,
-                                         TileSharedN<32>& T, uint64_t* xs) {
+__device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t nblk, int tid, R16Shared& T,
+                                         uint64_t* xs) {
     constexpr int W = 1 << H;
     constexpr int S = 32;               // sub-problems per tile
     constexpr int SP = R16Lds<H, LM>::SP;
@@ -868,9 +877,17 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
         for (int e = 0; e < 16; ++e) x[e] = 0;
     }
 
+    // destination rows of a row map shared by the tile's columns (S_bot = 1: one
+    // window base row for the tile), requested behind the data; the inverse's
+    // last pass (the key-switch's Galois scatter) is the pass that has one
+    const bool rows_lds = INV && P.dst_map && !P.map_bitrev && !P.dst_map_limb && P.S_bot == 1;
+    int32_t my_row = 0;
+    if (rows_lds && tid < W) my_row = P.dst_map[r16_sub(P, blk, nblk, 0).jbase + ((int64_t)tid << P.b0)];
+
     const ulonglong2* twrow = (const ulonglong2*)(P.tw + 2 * (int64_t)limb * P.m);
     // ---- first stages, exchange, remaining stages
     if (valid) r16_stages_any<H, INV, L_IN>(x, lane, twrow, top, P.logm, P.stage0, L, mode);
+    if (rows_lds && tid < W) T.rows[tid] = my_row;
     uint64_t* xs_s = xs + s * SP;
 #pragma unroll
     for (int e = 0; e < 16; ++e) xs_s[R16Lds<H, LM>::slot(r16_w<H>(L_IN, e, lane))] = x[e];
@@ -917,7 +934,8 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const int64_t j = jbase + ((int64_t)r16_w<H>(L_OUT, e, lane) << P.b0);
-        const int64_t row = P.map_bitrev ? (int64_t)(__brev((uint32_t)j) >> (32 - P.logm)) : (int64_t)map[j];
+        const int64_t row = P.map_bitrev ? (int64_t)(__brev((uint32_t)j) >> (32 - P.logm))
+                                         : (rows_lds ? (int64_t)T.rows[r16_w<H>(L_OUT, e, lane)] : (int64_t)map[j]);
         char* ptr = (char*)P.dst.ptr + es * (colbase + row * P.dst.row_stride);
         if (es == 4) *(int32_t*)ptr = (int32_t)x[e];
         else *(uint64_t*)ptr = x[e];
@@ -928,7 +946,7 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
 template <int H, bool INV, bool LM = false>
 __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
     extern __shared__ uint64_t xs[];    // [32][SP]
-    __shared__ TileSharedN<32> T;
+    __shared__ R16Shared T;
     r16_tile<H, INV, LM>(P, blk, nblk, threadIdx.x, T, xs);
 }
 
@@ -1074,7 +1092,7 @@ template <bool INV>
 __global__ __launch_bounds__(512, 4) void k_ntt_pass_r16_group(const PassArgs* list, const int64_t* starts,
                                                                int count) {
     extern __shared__ uint64_t xs[];
-    __shared__ TileSharedN<32> T[4];
+    __shared__ R16Shared T[4];
     const int i = instance_of(starts, count, blockIdx.x);
     const int64_t b = (int64_t)blockIdx.x - starts[i];
     const PassArgs& P = list[i];
