@@ -112,6 +112,8 @@ _PROTOS = {
     "hx_keyswitch_exec_ex": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, I64, I64, ctypes.c_void_p, VP,
                              STREAM],
     "hx_keyswitch_row_map": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
+    "hx_keyswitch_exec_rmac": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, I64, I64, ctypes.c_void_p, VP,
+                               VP, VP, I64, STREAM],
     "hx_keyswitch_group_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), VP,
                                   ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), VP, ctypes.c_int,
                                   STREAM],
@@ -480,6 +482,17 @@ class KeySwitchPlan:
                                        row_map.data_ptr() if row_map is not None else None, view(out),
                                        stream_of(out)))
         return out
+
+    def exec_rmac(self, x, key, acc, ct, diags, diag: int, key_limbs: int, key_count: int, row_map=None):
+        """exec_ex of x fused with one rotate-multiply-accumulate step into acc
+        (hx_keyswitch_exec_rmac): acc_0 += pt * (KS_0 + ct_0 permuted), acc_1 +=
+        pt * KS_1, pt the diagonal `diag` of diags [B, M, M, k]."""
+        _need_cuda(x, key, acc, ct, diags)
+        check(lib.hx_keyswitch_exec_rmac(self._h, view(x), key.data_ptr(), key.numel() * key.element_size(),
+                                         int(key_limbs), int(key_count),
+                                         row_map.data_ptr() if row_map is not None else None, view(acc), view(ct),
+                                         view(diags), int(diag), stream_of(acc)))
+        return acc
 
     def row_map(self, perm):
         """int32 [m] row map of one Galois permutation perm [m]."""

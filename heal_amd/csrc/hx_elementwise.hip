@@ -125,15 +125,6 @@ __global__ __launch_bounds__(kBlock) void k_modop(ElemArgs A) {
 // work-items touch adjacent 16-byte words.
 constexpr int kSeg = 2;
 
-__device__ __forceinline__ uint64_t modop_fast(int op, const ModInfo& mi, int64_t av, int64_t bv) {
-    if (mi.M == 0) return 0;
-    if ((uint64_t)av < mi.M && (uint64_t)bv < mi.M) {
-        if (op == 0) return mulmod(mi, (uint64_t)av, (uint64_t)bv);
-        const uint64_t s2 = (uint64_t)av + (uint64_t)bv;
-        return s2 >= mi.M ? s2 - mi.M : s2;
-    }
-    return (uint64_t)(op == 0 ? cpp_mulmod(mi, av, bv) : cpp_addmod(mi, av, bv));
-}
 
 template <typename T, int OP, int BSRC>
 __global__ __launch_bounds__(kBlock) void k_modop_rows(ElemArgs A, FastDiv segs_fd, uint32_t segs, uint32_t nitems) {

@@ -52,6 +52,23 @@ bool views_overlap(const hx_view* a, const hx_view* b);
 // Granlund & Montgomery, "Division by invariant integers using
 // multiplication", PLDI 1994 (round-up variant): for n < 2^31,
 // n / d = (umulhi(n, mul) + n) >> shift.
+#if defined(__HIPCC__)
+// The reference's modmul (op 0) / modsum (op 1) of one element pair,
+// (T_DP(a) op T_DP(b)) % p with |p| described by mi (example_impl/modop_impl.cpp:
+// 123-126): operands that are residues (0 <= x < |p|) take one double-width
+// product and reduction, or one conditional subtraction; any other pair takes
+// the truncating signed semantics.  A modulus of 0 gives 0.
+__device__ __forceinline__ uint64_t modop_fast(int op, const ModInfo& mi, int64_t av, int64_t bv) {
+    if (mi.M == 0) return 0;
+    if ((uint64_t)av < mi.M && (uint64_t)bv < mi.M) {
+        if (op == 0) return mulmod(mi, (uint64_t)av, (uint64_t)bv);
+        const uint64_t s2 = (uint64_t)av + (uint64_t)bv;
+        return s2 >= mi.M ? s2 - mi.M : s2;
+    }
+    return (uint64_t)(op == 0 ? cpp_mulmod(mi, av, bv) : cpp_addmod(mi, av, bv));
+}
+#endif
+
 struct FastDiv {
     uint32_t d;
     uint32_t mul;

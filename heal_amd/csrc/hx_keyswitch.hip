@@ -821,16 +821,54 @@ static void keyswitch_launch(KsArgs& K, hipStream_t st, void* const* marks = nul
 
 void keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                     int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, hipStream_t st,
-                    void* const* marks = nullptr) {
+                    void* const* marks = nullptr, const RmacArgs* rm = nullptr) {
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(x, "x", 3);
     if (x->dims[0] == 0) return;
     std::lock_guard<std::mutex> lk(plan->wss->mu);
     const Workspaces::Ws w = plan->wss->acquire(st, 0, keyswitch_ws_bytes(plan, x->dims[0]));
     KsArgs K;
-    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, w.data, K))
+    if (keyswitch_build(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, w.data, K)) {
+        if (rm) K.P3.rm = *rm;
         keyswitch_launch(K, st, marks);
+    }
     plan->wss->note(st);
+}
+
+// The key-switch of ct's second polynomial with the rotate-multiply-accumulate
+// of the serving loop in its last kernel (RmacArgs): acc is updated in place
+// and the key-switch output is never written.  acc and ct are [B, 2, m, k]
+// with equal strides, diags [B, M, M, k] contiguous per instance (M a power of
+// two); the moduli are the plan's.
+void keyswitch_exec_rmac(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                         int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* acc,
+                         const hx_view* ct, const hx_view* diags, int64_t diag, hipStream_t st) {
+    if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
+    require_i64(acc, "acc", 4);
+    require_i64(ct, "ct", 4);
+    require_i64(diags, "diags", 4);
+    for (int d = 0; d < 4; ++d) {
+        if (ct->dims[d] != acc->dims[d] || (ct->dims[d] > 1 && ct->strides[d] != acc->strides[d]))
+            raise(HX_ERR_INVALID_ARGUMENT, "ct must have the shape and strides of acc");
+    }
+    const int64_t B = acc->dims[0], M = diags->dims[1], k = plan->k;
+    if (diags->dims[0] != B || diags->dims[2] != M || diags->dims[3] != k)
+        raise(HX_ERR_INVALID_ARGUMENT, "diags must have shape [B, M, M, k]");
+    if (M <= 0 || (M & (M - 1)) != 0) raise(HX_ERR_INVALID_ARGUMENT, "the matrix size M must be a power of two");
+    if ((k > 1 && diags->strides[3] != 1) || (M > 1 && diags->strides[2] != k) || (M > 1 && diags->strides[1] != M * k))
+        raise(HX_ERR_INVALID_ARGUMENT, "diags must be contiguous within an instance");
+    if (diag < 0) raise(HX_ERR_INVALID_ARGUMENT, "diag must be >= 0");
+    if (views_overlap(acc, ct) || views_overlap(acc, diags) || views_overlap(acc, x))
+        raise(HX_ERR_INVALID_ARGUMENT, "acc must not overlap the other operands");
+    RmacArgs rm;
+    rm.on = 1;
+    rm.ct = (const int64_t*)ct->data;
+    rm.diags = (const int64_t*)diags->data;
+    rm.diags_b = diags->strides[0];
+    rm.M = M;
+    rm.diag = diag % M;
+    rm.L = k;
+    keyswitch_exec(plan, x, key, key_bytes, key_limbs, key_count, row_map, acc, st, nullptr, &rm);
 }
 
 // ------------------------------------------------------------ autotuning
@@ -1231,6 +1269,16 @@ int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const 
     HX_API_BEGIN
     if (key_limbs < 1 || key_count < 1) raise(HX_ERR_INVALID_ARGUMENT, "key_limbs and key_count must be positive");
     keyswitch_exec(plan, x, key, key_bytes, key_limbs, key_count, row_map, out, as_stream(stream));
+    HX_API_END
+}
+
+int hx_keyswitch_exec_rmac(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                           int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* acc,
+                           const hx_view* ct, const hx_view* diags, int64_t diag, hx_stream stream) {
+    HX_API_BEGIN
+    if (key_limbs < 1 || key_count < 1) raise(HX_ERR_INVALID_ARGUMENT, "key_limbs and key_count must be positive");
+    keyswitch_exec_rmac(plan, x, key, key_bytes, key_limbs, key_count, row_map, acc, ct, diags, diag,
+                        as_stream(stream));
     HX_API_END
 }
 

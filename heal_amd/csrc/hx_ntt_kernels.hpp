@@ -200,6 +200,21 @@ struct ColAddr {
                    // 3: int64 tensor read as base-2^w digits (fused g_decomposition)
 };
 
+// Store epilogue of the key-switch's last pass for the serving loop's
+// rotate-multiply-accumulate: instead of writing the key-switch output KS, the
+// pass updates dst (the accumulator, [B, 2, m, k]) in place,
+//     dst_c[row] = (dst_c[row] + rot_c * pt[row]) mod q_t,
+//     rot_0 = (KS_0 + ct_0[j]) mod q_t,  rot_1 = KS_1,
+// where row is the Galois row the key-switch writes coefficient j to, ct has
+// dst's layout, and pt[row] = diags[b, diag, row mod M, t] ([B, M, M, k],
+// contiguous per instance).  Each step is the reference op's own (modop_fast).
+struct RmacArgs {
+    int32_t on;
+    const int64_t* ct;
+    const int64_t* diags;
+    int64_t diags_b, M, diag, L;  // L: the limbs of diags (its innermost dim)
+};
+
 struct PassArgs {
     ColAddr src;
     ColAddr dst;
@@ -220,6 +235,7 @@ struct PassArgs {
     // n_bot_tiles, cdim[0], cdim[1]; S_cols = 2^log_scols
     FastDiv fd_ctiles, fd_btiles, fd_c0, fd_c1;
     int32_t log_scols;
+    RmacArgs rm;             // store epilogue of the key-switch's last pass (rm.on)
     int32_t reduce_in;
     int32_t scale_out;
     int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 8q) forward, [0, 4q) inverse)
@@ -700,6 +716,7 @@ struct R16Lds {
 struct R16Sub {
     int64_t src_off, dst_off, alt_off, jbase, col, top;
     int32_t limb, digit, valid, skip;
+    int32_t cc0, cc2;  // column coordinates 0 and 2
 };
 
 // Sub-problem s of tile `tile` (already mapped to the tile order; in_range
@@ -725,6 +742,8 @@ __device__ __forceinline__ R16Sub r16_sub_at(const PassArgs& P, uint32_t tile, b
     u.alt_off = P.alt.ptr ? col_offset(P.alt, coord) : 0;
     u.limb = (int32_t)coord[P.limb_axis];
     u.digit = P.src.kind == 3 ? (int32_t)coord[P.gd_axis] : 0;
+    u.cc0 = (int32_t)coord[0];
+    u.cc2 = (int32_t)coord[2];
     u.valid = valid;
     u.top = top;
     u.jbase = ((int64_t)top << (P.b0 + P.H)) + (valid ? bot : 0);
@@ -751,7 +770,11 @@ struct R16Shared {
 // Tile `blk` of a radix-16 pass's `nblk` tiles, served by the 32 * 2^(H-4)
 // threads tid = 0.. of a workgroup with tile bookkeeping T and an LDS block xs
 // of [32][SP] slots.  Every thread of the workgroup runs the same barriers.
-template <int H, bool INV, bool LM = false>
+// values per load group of the accumulate epilogue: four keeps the H = 8 instance
+// within 128 VGPRs (four waves per SIMD) without spills
+constexpr int kRmGroup = 4;
+
+template <int H, bool INV, bool LM = false, bool RM = false>
 __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t nblk, int tid, R16Shared& T,
                                          uint64_t* xs) {
     constexpr int W = 1 << H;
@@ -931,6 +954,38 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
         return;
     }
     const int32_t* map = P.dst_map_limb ? P.dst_map_limb + (int64_t)limb * P.m : P.dst_map;
+    if constexpr (RM) {
+        // rotate-multiply-accumulate into dst (the key-switch's last pass only)
+        const ModInfo mi = limb_modinfo(L);
+        const int64_t pt_base = (int64_t)u.cc2 * P.rm.diags_b + P.rm.diag * P.rm.M * P.rm.L + limb;
+        const int64_t mmask = P.rm.M - 1;
+        const int64_t* acc = (const int64_t*)P.dst.ptr;
+        // the loads of G values are issued before their stores: a store may alias a later load as far as the
+        // compiler knows, so loads interleaved with stores would wait one HBM round trip per value
+        constexpr int G = kRmGroup;
+#pragma unroll
+        for (int h = 0; h < 16; h += G) {
+            int64_t ao[G], av[G], cv[G], pv[G];
+#pragma unroll
+            for (int e = 0; e < G; ++e) {
+                const int w = r16_w<H>(L_OUT, h + e, lane);
+                const int64_t j = jbase + ((int64_t)w << P.b0);
+                const int64_t row = P.map_bitrev ? (int64_t)(__brev((uint32_t)j) >> (32 - P.logm))
+                                                 : (rows_lds ? (int64_t)T.rows[w] : (int64_t)map[j]);
+                ao[e] = colbase + row * P.dst.row_stride;
+                av[e] = acc[ao[e]];
+                cv[e] = u.cc0 == 0 ? P.rm.ct[colbase + j * P.dst.row_stride] : 0;
+                pv[e] = P.rm.diags[pt_base + (row & mmask) * P.rm.L];
+            }
+#pragma unroll
+            for (int e = 0; e < G; ++e) {
+                int64_t rot = (int64_t)x[h + e];
+                if (u.cc0 == 0) rot = (int64_t)modop_fast(1, mi, rot, cv[e]);
+                ((int64_t*)P.dst.ptr)[ao[e]] = (int64_t)modop_fast(1, mi, av[e], (int64_t)modop_fast(0, mi, rot, pv[e]));
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const int64_t j = jbase + ((int64_t)r16_w<H>(L_OUT, e, lane) << P.b0);
@@ -943,16 +998,18 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
 }
 
 // One workgroup of a radix-16 pass: tile `blk` of the pass's `nblk` tiles.
-template <int H, bool INV, bool LM = false>
+template <int H, bool INV, bool LM = false, bool RM = false>
 __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
     extern __shared__ uint64_t xs[];    // [32][SP]
     __shared__ R16Shared T;
-    r16_tile<H, INV, LM>(P, blk, nblk, threadIdx.x, T, xs);
+    r16_tile<H, INV, LM, RM>(P, blk, nblk, threadIdx.x, T, xs);
 }
 
-template <int H, bool INV, bool LM>
-__global__ __launch_bounds__(32 << (H - 4)) void k_ntt_pass_r16(PassArgs P) {
-    r16_pass<H, INV, LM>(P, blockIdx.x, gridDim.x);
+// RM: the inverse pass whose store is the rotate-multiply-accumulate epilogue
+// (P.rm), an instance of its own so that the plain passes keep their register budget
+template <int H, bool INV, bool LM, bool RM>
+__global__ __launch_bounds__(32 << (H - 4), RM && H == 8 ? 4 : 1) void k_ntt_pass_r16(PassArgs P) {
+    r16_pass<H, INV, LM, RM>(P, blockIdx.x, gridDim.x);
 }
 
 
@@ -1363,6 +1420,8 @@ void launch_pass(PassArgs& P, hipStream_t st) {
     const int W = 1 << P.H;
     const int S = P.S_cols * P.S_bot;
     const int64_t blocks = pass_blocks(P);
+    if (P.rm.on && (P.top_bitrev || !r16_window(P.H)))
+        raise(HX_ERR_RUNTIME, "internal: accumulate epilogue needs a radix-16 pass without bit-reversal tiles");
     if (P.top_bitrev) {
         if (!r16_window(P.H) || P.b0 != 0 || P.logm - P.H < 5) raise(HX_ERR_RUNTIME, "internal: bit-reversal tiles");
         const size_t lds = (size_t)kR16Tile * (W + 1) * 8;
@@ -1387,9 +1446,14 @@ void launch_pass(PassArgs& P, hipStream_t st) {
         const size_t lds = (size_t)kR16Tile * (lm ? W + W / 16 : W + 1) * 8;
         const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
 #define HX_R16(HH, LMV)                                                                              \
-    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true, LMV>), grid, block, lds, st, P);     \
-    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false, LMV>), grid, block, lds, st, P);
-        if (P.H == 8) {
+    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true, LMV, false>), grid, block, lds, st, P); \
+    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false, LMV, false>), grid, block, lds, st, P);
+        if (P.rm.on) {
+            if (!P.inverse || lm) raise(HX_ERR_RUNTIME, "internal: accumulate epilogue on a forward or lane-major pass");
+            if (P.H == 8) hipLaunchKernelGGL((k_ntt_pass_r16<8, true, false, true>), grid, block, lds, st, P);
+            else if (P.H == 7) hipLaunchKernelGGL((k_ntt_pass_r16<7, true, false, true>), grid, block, lds, st, P);
+            else hipLaunchKernelGGL((k_ntt_pass_r16<6, true, false, true>), grid, block, lds, st, P);
+        } else if (P.H == 8) {
             if (lm) {
                 HX_R16(8, true)
             } else {

@@ -25,9 +25,9 @@ Serving model:
   request's share;
 * each pool cycles through the rotation indices; one step runs rotation r
   for all admitted requests of the pool: one batched key-switch (shared key
-  and row map) and one fused rotate-multiply-accumulate launch
-  (hx_rot_mac).  A request admitted at rotation r finishes after R steps of
-  its pool.  Modular addition is exact and commutative, so a request's
+  and row map) whose last kernel applies the rotate-multiply-accumulate to the
+  accumulators (hx_keyswitch_exec_rmac; rotation 0 is one hx_rot_mac).  A
+  request admitted at rotation r finishes after R steps of its pool.  Modular addition is exact and commutative, so a request's
   result does not depend on when it joined or on the other requests;
 * every pool runs on its own stream (default priority) and the host never
   waits for the device: it enqueues up to `depth` steps per pool ahead,
@@ -96,8 +96,8 @@ class Request:
 
     @property
     def bytes(self) -> int:
-        # ct, acc, ks out: 2NL each (int64), plus the matrix
-        return (6 * self.n + MAT * MAT) * self.limbs * 8
+        # ct and acc: 2NL each (int64), plus the matrix
+        return (4 * self.n + MAT * MAT) * self.limbs * 8
 
 
 @dataclass
@@ -128,7 +128,6 @@ class _Ring:
         self.perms = [None] * rotations
         for r in range(1, rotations):
             self.perms[r] = P.galois_perms(n, pow(5, r, 2 * n), 1, device=device).expand(max_limbs, n)
-        self.perm_rows = [None] + [self.perms[r][0].contiguous() for r in range(1, rotations)]
         full = self.plan(max_limbs)[2]
         self.row_maps = [None] + [full.row_map(self.perms[r][0]) for r in range(1, rotations)]
         self.keys = [None] + [full.prepare_key(self.evk(r)) for r in range(1, rotations)]
@@ -166,7 +165,6 @@ class _Pool:
         with torch.cuda.stream(self.stream):
             self.ct = torch.zeros(max_slots, 2, n, L, dtype=torch.int64, device=dev)
             self.acc = torch.zeros_like(self.ct)
-            self.out = torch.zeros_like(self.ct)
             # the matrices by diagonals: diags[s, r, i, t] = M_s[i, (i + r) mod MAT] mod q_t
             self.diags = torch.zeros(max_slots, MAT, MAT, L, dtype=torch.int64, device=dev)
 
@@ -195,9 +193,9 @@ class _Pool:
             if r == 0:
                 capi.rot_mac(self.acc[:B], None, self.ct[:B], None, self.diags[:B], 0, self.q)
             else:
-                self.ks.exec_ex(self.ct[:B, 1], ring.keys[r], self.out[:B], key_limbs=ring.max_limbs, key_count=1,
-                                row_map=ring.row_maps[r])
-                capi.rot_mac(self.acc[:B], self.out[:B], self.ct[:B], ring.perm_rows[r], self.diags[:B], r, self.q)
+                # the key-switch with the rotate-multiply-accumulate in its last kernel
+                self.ks.exec_rmac(self.ct[:B, 1], ring.keys[r], self.acc[:B], self.ct[:B], self.diags[:B], r,
+                                  key_limbs=ring.max_limbs, key_count=1, row_map=ring.row_maps[r])
             rec.end.record()
             rec.enqueue_s = time.perf_counter() - h0
             for q in self.reqs:

@@ -315,6 +315,20 @@ int hx_keyswitch_exec_marked(const hx_keyswitch_plan* plan, const hx_view* x, co
 int hx_keyswitch_exec_ex(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                          int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out,
                          hx_stream stream);
+/* hx_keyswitch_exec_ex of x (the second polynomial of each instance's
+ * ciphertext ct) fused with one step of the diagonal matrix-vector product of
+ * the serving loop: instead of writing the key-switch output KS, its last
+ * kernel updates acc [B, 2, m, k] in place,
+ *     acc_0[j] += pt[j] * (KS_0[j] + ct_0[perm[j]]),  acc_1[j] += pt[j] * KS_1[j]
+ * (mod the plan's moduli, each step with the semantics of hx_rot_mac), where
+ * perm is the Galois permutation row_map inverts and pt[j] = diags[b, diag mod
+ * M, j mod M, t].  ct has acc's shape and strides; diags is [B, M, M, k],
+ * contiguous per instance, M a power of two.  Equal to hx_keyswitch_exec_ex
+ * followed by hx_rot_mac, without writing and re-reading KS.  No counterpart in
+ * the reference (a vendor extension for serving). */
+int hx_keyswitch_exec_rmac(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
+                           int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* acc,
+                           const hx_view* ct, const hx_view* diags, int64_t diag, hx_stream stream);
 /* Exact negacyclic NTT of length 65536 for primes q < 2^28 on the INT8 matrix
  * cores (four-step: 256-point sub-transforms as 256 x 256 x 256 products of
  * 7-bit int8 planes with int32 accumulation, recombined and reduced mod q).

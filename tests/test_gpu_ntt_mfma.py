@@ -74,3 +74,37 @@ def test_rejects_large_primes(gpu):
     q = keyswitch_primes(1)[0]
     with pytest.raises(ValueError):
         tables([q], [primitive_root_2n(q, N)])
+
+
+def test_config4_full_size_32_primes(gpu):
+    """The exact matrix-core configuration: 16 polynomials x 32 primes below
+    2^28, N = 65536, limb-major, seeded uniform residues; bit-exact with the
+    ALU NTT on every column and with the oracle's network on two columns."""
+    from heal_amd import capi
+    from heal_amd.ntt_mfma import NttMfmaPlan, small_primes
+    from heal_amd.params import NttParams
+
+    print("[mfma] config 4: 16 x 32 primes", flush=True)
+    primes = small_primes(32)
+    assert len(primes) == 32 and max(primes) < (1 << 28)
+    prm = NttParams(N, primes, device="cuda")
+    mfma = NttMfmaPlan(primes, prm.psi)
+    alu = capi.NttPlan(prm.p, prm.perm, prm.twiddles)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    data = torch.randint(0, 1 << 62, (16, 32, N), generator=g, device="cuda") % prm.p.view(1, -1, 1)
+    a = data.permute(0, 2, 1).unsqueeze(2)  # [16, N, 1, 32], limb-major in memory
+    o1 = torch.zeros_like(data).permute(0, 2, 1).unsqueeze(2)
+    o2 = torch.ones_like(data).permute(0, 2, 1).unsqueeze(2)
+    mfma(a, o1)
+    alu(a, o2)
+    torch.cuda.synchronize()
+    assert torch.equal(o1, o2)
+    o = OracleBackend()
+    for b, t in ((0, 0), (15, 31)):
+        col = a[b:b + 1, :, :, t:t + 1].cpu().contiguous()
+        exp = torch.zeros_like(col)
+        assert o.ntt(col, prm.p[t:t + 1].cpu(), prm.perm.cpu(), prm.twiddles[t:t + 1].cpu(), exp) is None
+        assert torch.equal(o1[b:b + 1, :, :, t:t + 1].cpu(), exp), (b, t)
+    mfma.close()
+    alu.close()

@@ -157,6 +157,50 @@ def test_rot_mac_equals_the_five_calls(gpu):
         capi.rot_mac(got, dev["ks"], dev["ct"], None, dev["diags"], diag, dev["q"])
 
 
+@pytest.mark.parametrize("B,n,L", [(3, 4096, 4), (2, 65536, 24)])
+def test_keyswitch_rmac_equals_keyswitch_then_rot_mac(gpu, B, n, L):
+    """hx_keyswitch_exec_rmac (the rotate-multiply-accumulate in the key-switch's
+    last kernel) against hx_keyswitch_exec_ex followed by hx_rot_mac, with
+    signed and unreduced accumulators and diagonals; shared key, per-call row
+    map, two rotations of one plan.  The two ring sizes run the last pass with
+    64- and 256-point windows."""
+    from heal_amd import capi
+    from heal_amd import params as P
+
+    M, D = 128, 3
+    prm = P.NttParams(n, P.keyswitch_primes(L), device="cuda")
+    fwd = capi.NttPlan(prm.p, prm.perm, prm.twiddles)
+    inv = capi.NttPlan(prm.p, prm.perm, prm.inv_twiddles, prm.m_inv, inverse=True)
+    ks = capi.KeySwitchPlan(fwd, inv, P.galois_perms(n, 5, L, device="cuda"), D, 60)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(21)
+    q = prm.p
+    ct = torch.randint(0, 1 << 62, (B, 2, n, L), generator=g, device="cuda") % q
+    evk = torch.randint(0, 1 << 62, (D, 1, 2, n, L), generator=g, device="cuda") % q
+    key = ks.prepare_key(evk)
+    acc0 = torch.randint(-(1 << 62), 1 << 62, (B, 2, n, L), generator=g, device="cuda")
+    acc0[0] %= q
+    diags = torch.randint(-(1 << 62), 1 << 62, (B, M, M, L), generator=g, device="cuda")
+    diags[1] %= q
+    for r, diag in ((3, 3), (5, 133)):
+        print(f"[serving] exec_rmac N={n} L={L} rotation {r}", flush=True)
+        perm = P.galois_perms(n, pow(5, r, 2 * n), 1, device="cuda")[0].contiguous()
+        rmap = ks.row_map(perm)
+        out = torch.empty_like(ct)
+        ks.exec_ex(ct[:, 1], key, out, key_limbs=L, key_count=1, row_map=rmap)
+        want = acc0.clone()
+        capi.rot_mac(want, out, ct, perm, diags, diag, q)
+        got = acc0.clone()
+        ks.exec_rmac(ct[:, 1], key, got, ct, diags, diag, key_limbs=L, key_count=1, row_map=rmap)
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), r
+    with pytest.raises(ValueError):  # acc must not overlap ct
+        ks.exec_rmac(ct[:, 1], key, ct, ct, diags, 3, key_limbs=L, key_count=1, row_map=rmap)
+    ks.close()
+    fwd.close()
+    inv.close()
+
+
 def test_n16384_l8_eight_rotations_against_the_oracle(gpu):
     """A mid-size ring (N = 2^14, L = 8, 8 rotations, w = 60): two requests
     served together, each against the oracle."""
