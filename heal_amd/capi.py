@@ -103,7 +103,8 @@ _PROTOS = {
     "hx_keyswitch_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p, VP, U64, U64,
                                  STREAM],
     "hx_keyswitch_key_bytes": [ctypes.c_void_p, I64, ctypes.POINTER(ctypes.c_size_t)],
-    "hx_keyswitch_plan_config": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
+    "hx_keyswitch_plan_config": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int32)],
     "hx_keyswitch_prepare_key": [ctypes.c_void_p, VP, ctypes.c_void_p, STREAM],
     "hx_keyswitch_exec": [ctypes.c_void_p, VP, ctypes.c_void_p, ctypes.c_size_t, VP, STREAM],
     "hx_keyswitch_plan_destroy": [ctypes.c_void_p],
@@ -444,10 +445,11 @@ class KeySwitchPlan:
                                            self.base_bits, stream_of(perms)))
 
     def config(self) -> dict:
-        """The tuned shape: index bits of the middle kernel and its limb group."""
-        lo, g = ctypes.c_int32(), ctypes.c_int32()
-        check(lib.hx_keyswitch_plan_config(self._h, ctypes.byref(lo), ctypes.byref(g)))
-        return {"low_bits": lo.value, "limb_group": g.value}
+        """The tuned shape: index bits of the middle kernel, its limb group, and
+        the columns per workgroup of the first and last kernels."""
+        lo, g, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib.hx_keyswitch_plan_config(self._h, ctypes.byref(lo), ctypes.byref(g), ctypes.byref(t)))
+        return {"low_bits": lo.value, "limb_group": g.value, "tile": t.value}
 
     def prepare_key(self, evk):
         """evk [dnum, B, 2, m, k] -> device-layout key (an int64 tensor)."""

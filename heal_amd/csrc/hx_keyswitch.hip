@@ -40,6 +40,7 @@ struct hx_keyswitch_plan {
     int logm, hhi, hlo;
     int D, w;
     int G;            // limbs per K2 workgroup (tuned)
+    int tile;         // sub-problems per K1/K3 workgroup, 32 or 16 (tuned)
     void* mem;
     int32_t* galmap;  // [k][m]: output row of coefficient j of limb t
     int gal_shared;   // every limb has the same map (K3 then reads row 0 only)
@@ -480,6 +481,7 @@ hx_keyswitch_plan* keyswitch_plan_create(const hx_ntt_plan* fwd, const hx_ntt_pl
     plan->D = (int)dnum;
     plan->w = (int)base_bits;
     plan->G = 1;
+    plan->tile = 32;
     const size_t map_bytes = ((size_t)k * m * 4 + 255) / 256 * 256;
     check_hip(hipMallocAsync(&plan->mem, 2 * map_bytes + 256, st), "hipMallocAsync(keyswitch plan)");
     plan->galmap = (int32_t*)plan->mem;
@@ -669,7 +671,8 @@ size_t keyswitch_ws_bytes(const hx_keyswitch_plan* plan, int64_t B) {
 // empty batch.
 bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                      int64_t key_limbs, int64_t key_count, const int32_t* row_map, const hx_view* out, void* ws,
-                     KsArgs& K) {
+                     KsArgs& K, int tile = 0) {
+    if (plan && !tile) tile = plan->tile;
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     require_i64(x, "x", 3);
     require_i64(out, "out", 4);
@@ -729,7 +732,7 @@ bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void
     P1.limbs = F->limbs;
     P1.tw = F->tw;
     P1.m = m;
-    configure_tile(P1, true, 2);
+    configure_tile(P1, true, 2, tile);
     K.blocks1 = pass_blocks(P1);
     K.H1 = plan->hhi;
 
@@ -792,7 +795,7 @@ bool keyswitch_build(const hx_keyswitch_plan* plan, const hx_view* x, const void
     P3.limbs = I->limbs;
     P3.tw = I->tw;
     P3.m = m;
-    configure_tile(P3, true, 2);
+    configure_tile(P3, true, 2, tile);
     K.blocks3 = pass_blocks(P3);
     return true;
 }
@@ -873,7 +876,8 @@ void keyswitch_exec_rmac(const hx_keyswitch_plan* plan, const hx_view* x, const 
 
 // ------------------------------------------------------------ autotuning
 // Per ring degree m and dnum: the split of log m into the K1/K3 window (high
-// bits) and the K2 window (low bits); per (m, k, dnum): the K2 limb group G.
+// bits) and the K2 window (low bits); per (m, k, dnum): the K2 limb group G
+// and the K1/K3 tile (32 or 16 columns per workgroup).
 // Every choice computes the same result; they differ in speed only.  The split
 // is chosen once per (m, dnum) in a process, so keys prepared by one plan serve
 // every plan of that ring degree.  Choices are kept in a text file keyed by
@@ -885,6 +889,7 @@ void keyswitch_exec_rmac(const hx_keyswitch_plan* plan, const hx_view* x, const 
 static std::mutex g_tune_mu;
 static std::map<std::pair<int64_t, int>, int> g_tuned_hlo;
 static std::map<std::tuple<int64_t, int64_t, int>, int> g_tuned_group;
+static std::map<std::tuple<int64_t, int64_t, int>, int> g_tuned_tile;
 static bool g_tune_file_read = false;
 
 static std::string tune_cache_path() {
@@ -901,8 +906,9 @@ static std::string gpu_arch(int dev) {
     return c == std::string::npos ? a : a.substr(0, c);
 }
 
-// Lines "arch m k dnum low_bits limb_group"; entries of other architectures
-// and entries a process has already decided are skipped.  Caller holds g_tune_mu.
+// Lines "arch m k dnum low_bits limb_group tile" (a line without the tile
+// means 32); entries of other architectures, malformed lines and entries a
+// process has already decided are skipped.  Caller holds g_tune_mu.
 static void tune_file_read(const std::string& arch) {
     if (g_tune_file_read) return;
     g_tune_file_read = true;
@@ -910,19 +916,24 @@ static void tune_file_read(const std::string& arch) {
     if (path.empty()) return;
     FILE* f = fopen(path.c_str(), "r");
     if (!f) return;
-    char a[64];
-    long long m, k;
-    int D, hlo, G;
-    while (fscanf(f, "%63s %lld %lld %d %d %d", a, &m, &k, &D, &hlo, &G) == 6) {
-        if (arch != a || hlo < 6 || hlo > 8 || G < 1 || G > 8) continue;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        char a[64];
+        long long m, k;
+        int D, hlo, G, tile = 32;
+        const int n = sscanf(line, "%63s %lld %lld %d %d %d %d", a, &m, &k, &D, &hlo, &G, &tile);
+        if (n < 6 || arch != a || hlo < 6 || hlo > 8 || G < 1 || G > 8 || (tile != 16 && tile != 32)) continue;
         g_tuned_hlo.emplace(std::make_pair((int64_t)m, D), hlo);
-        // a group is kept only with the split it was timed for
-        if (g_tuned_hlo[{(int64_t)m, D}] == hlo) g_tuned_group.emplace(std::make_tuple((int64_t)m, (int64_t)k, D), G);
+        // a group and a tile are kept only with the split they were timed for
+        if (g_tuned_hlo[{(int64_t)m, D}] == hlo) {
+            g_tuned_group.emplace(std::make_tuple((int64_t)m, (int64_t)k, D), G);
+            g_tuned_tile.emplace(std::make_tuple((int64_t)m, (int64_t)k, D), tile);
+        }
     }
     fclose(f);
 }
 
-static void tune_file_append(const std::string& arch, int64_t m, int64_t k, int D, int hlo, int G) {
+static void tune_file_append(const std::string& arch, int64_t m, int64_t k, int D, int hlo, int G, int tile) {
     const std::string path = tune_cache_path();
     if (path.empty()) return;
     // best effort: create the directories on the path
@@ -930,7 +941,7 @@ static void tune_file_append(const std::string& arch, int64_t m, int64_t k, int 
         (void)mkdir(path.substr(0, i).c_str(), 0755);
     FILE* f = fopen(path.c_str(), "a");
     if (!f) return;
-    fprintf(f, "%s %lld %lld %d %d %d\n", arch.c_str(), (long long)m, (long long)k, D, hlo, G);
+    fprintf(f, "%s %lld %lld %d %d %d %d\n", arch.c_str(), (long long)m, (long long)k, D, hlo, G, tile);
     fclose(f);
 }
 
@@ -967,17 +978,20 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     const std::string arch = gpu_arch(plan->device);
     tune_file_read(arch);
-    int hlo = -1, G = -1;
+    int hlo = -1, G = -1, tile = 32;
     {
         auto a = g_tuned_hlo.find({m, D});
         if (a != g_tuned_hlo.end() && std::find(hlos.begin(), hlos.end(), a->second) != hlos.end()) hlo = a->second;
         auto b = g_tuned_group.find({m, k, D});
         if (hlo >= 0 && b != g_tuned_group.end() && inner_fits(hlo, D, b->second)) G = b->second;
+        auto c = g_tuned_tile.find({m, k, D});
+        if (c != g_tuned_tile.end()) tile = c->second;
     }
     if (hlo >= 0 && G >= 0) {
         plan->hlo = hlo;
         plan->hhi = logm - hlo;
         plan->G = G;
+        plan->tile = tile;
         return;
     }
     // scratch key-switch: B instances of zeros (the timing does not depend on
@@ -1008,6 +1022,7 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
         ov.strides[0] = 2 * m * k, ov.strides[1] = m * k, ov.strides[2] = k, ov.strides[3] = 1;
         void* key = (char*)mem + xb + ob;
         void* ws = (char*)mem + xb + ob + kb;
+        plan->tile = 32;
         if (hlo < 0) {
             float best = 1e30f;
             for (int h : hlos) {
@@ -1028,6 +1043,14 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
             const float t = time_keyswitch(plan, &xv, key, kb, &ov, ws, st, e0, e1);
             if (t < best) best = t, G = g;
         }
+        // K1/K3 workgroups of 16 sub-problems (twice as many workgroups, each
+        // half the size) against the 32 timed above
+        if (G >= 0) {
+            plan->G = G;
+            plan->tile = 16;
+            if (time_keyswitch(plan, &xv, key, kb, &ov, ws, st, e0, e1) < best) tile = 16;
+            plan->tile = 32;
+        }
     } catch (...) {
         (void)hipStreamSynchronize(st);
         (void)hipFreeAsync(mem, st);
@@ -1041,9 +1064,11 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
     check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (G < 0 || !inner_fits(hlo, D, G)) raise(HX_ERR_INVALID_ARGUMENT, "dnum too large for the fused key-switch tile");
     plan->G = G;
+    plan->tile = tile;
     g_tuned_hlo[{m, D}] = hlo;
     g_tuned_group[{m, k, D}] = G;
-    tune_file_append(arch, m, k, D, hlo, G);
+    g_tuned_tile[{m, k, D}] = tile;
+    tune_file_append(arch, m, k, D, hlo, G, tile);
 }
 
 // ------------------------------------------------------------ grouped execution
@@ -1103,7 +1128,9 @@ hx_keyswitch_group* keyswitch_group_create(const hx_keyswitch_plan* const* plans
                 g->workspaces.push_back(ws);
             }
             KsArgs K;
-            const bool any = keyswitch_build(plans[i], &xs[i], keys[i], key_bytes[i], -1, -1, nullptr, &outs[i], ws, K);
+            // the group kernels serve 32-sub-problem tiles whatever the plan's tuned tile
+            const bool any =
+                keyswitch_build(plans[i], &xs[i], keys[i], key_bytes[i], -1, -1, nullptr, &outs[i], ws, K, kR16Tile);
             if (any) ks.push_back(K);
         }
     } catch (...) {
@@ -1234,11 +1261,12 @@ int hx_keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch, size_t*
     HX_API_END
 }
 
-int hx_keyswitch_plan_config(const hx_keyswitch_plan* plan, int32_t* low_bits, int32_t* limb_group) {
+int hx_keyswitch_plan_config(const hx_keyswitch_plan* plan, int32_t* low_bits, int32_t* limb_group, int32_t* tile) {
     HX_API_BEGIN
     if (!plan) raise(HX_ERR_INVALID_ARGUMENT, "plan pointer must not be null.");
     if (low_bits) *low_bits = plan->hlo;
     if (limb_group) *limb_group = plan->G;
+    if (tile) *tile = plan->tile;
     HX_API_END
 }
 

@@ -769,16 +769,17 @@ struct R16Shared {
 
 // Tile `blk` of a radix-16 pass's `nblk` tiles, served by the 32 * 2^(H-4)
 // threads tid = 0.. of a workgroup with tile bookkeeping T and an LDS block xs
-// of [32][SP] slots.  Every thread of the workgroup runs the same barriers.
+// of [S][SP] slots (S = 32 or 16 sub-problems).  Every thread of the workgroup
+// runs the same barriers.
 // values per load group of the accumulate epilogue: four keeps the H = 8 instance
 // within 128 VGPRs (four waves per SIMD) without spills
 constexpr int kRmGroup = 4;
 
-template <int H, bool INV, bool LM = false, bool RM = false>
+template <int H, bool INV, bool LM = false, bool RM = false, int S = 32>
 __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t nblk, int tid, R16Shared& T,
                                          uint64_t* xs) {
+    static_assert(S == 16 || S == 32, "sub-problems per tile");
     constexpr int W = 1 << H;
-    constexpr int S = 32;               // sub-problems per tile
     constexpr int SP = R16Lds<H, LM>::SP;
     constexpr int L_IN = INV ? 1 : 0;   // layout of the first stages
     constexpr int L_OUT = 1 - L_IN;
@@ -998,18 +999,19 @@ __device__ __forceinline__ void r16_tile(const PassArgs& P, int64_t blk, int64_t
 }
 
 // One workgroup of a radix-16 pass: tile `blk` of the pass's `nblk` tiles.
-template <int H, bool INV, bool LM = false, bool RM = false>
+template <int H, bool INV, bool LM = false, bool RM = false, int S = 32>
 __device__ __forceinline__ void r16_pass(const PassArgs& P, int64_t blk, int64_t nblk) {
-    extern __shared__ uint64_t xs[];    // [32][SP]
+    extern __shared__ uint64_t xs[];    // [S][SP]
     __shared__ R16Shared T;
-    r16_tile<H, INV, LM, RM>(P, blk, nblk, threadIdx.x, T, xs);
+    r16_tile<H, INV, LM, RM, S>(P, blk, nblk, threadIdx.x, T, xs);
 }
 
 // RM: the inverse pass whose store is the rotate-multiply-accumulate epilogue
-// (P.rm), an instance of its own so that the plain passes keep their register budget
-template <int H, bool INV, bool LM, bool RM>
-__global__ __launch_bounds__(32 << (H - 4), RM && H == 8 ? 4 : 1) void k_ntt_pass_r16(PassArgs P) {
-    r16_pass<H, INV, LM, RM>(P, blockIdx.x, gridDim.x);
+// (P.rm), an instance of its own so that the plain passes keep their register
+// budget.  S: sub-problems per workgroup (S * 2^(H-4) threads).
+template <int H, bool INV, bool LM, bool RM, int S>
+__global__ __launch_bounds__(S << (H - 4), RM && H == 8 && S == 32 ? 4 : 1) void k_ntt_pass_r16(PassArgs P) {
+    r16_pass<H, INV, LM, RM, S>(P, blockIdx.x, gridDim.x);
 }
 
 
@@ -1441,18 +1443,42 @@ void launch_pass(PassArgs& P, hipStream_t st) {
         return;
     }
     if (r16_window(P.H)) {
-        if (S != kR16Tile) raise(HX_ERR_RUNTIME, "internal: radix-16 tile size");
         const bool lm = P.lane_major != 0;
-        const size_t lds = (size_t)kR16Tile * (lm ? W + W / 16 : W + 1) * 8;
-        const dim3 grid((unsigned)blocks), block((unsigned)(kR16Tile * (W / 16)));
-#define HX_R16(HH, LMV)                                                                              \
-    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true, LMV, false>), grid, block, lds, st, P); \
-    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false, LMV, false>), grid, block, lds, st, P);
-        if (P.rm.on) {
-            if (!P.inverse || lm) raise(HX_ERR_RUNTIME, "internal: accumulate epilogue on a forward or lane-major pass");
-            if (P.H == 8) hipLaunchKernelGGL((k_ntt_pass_r16<8, true, false, true>), grid, block, lds, st, P);
-            else if (P.H == 7) hipLaunchKernelGGL((k_ntt_pass_r16<7, true, false, true>), grid, block, lds, st, P);
-            else hipLaunchKernelGGL((k_ntt_pass_r16<6, true, false, true>), grid, block, lds, st, P);
+        if (S != kR16Tile && !(S == 16 && !lm)) raise(HX_ERR_RUNTIME, "internal: radix-16 tile size");
+        const size_t lds = (size_t)S * (lm ? W + W / 16 : W + 1) * 8;
+        const dim3 grid((unsigned)blocks), block((unsigned)(S * (W / 16)));
+#define HX_R16(HH, LMV)                                                                                     \
+    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true, LMV, false, 32>), grid, block, lds, st, P); \
+    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false, LMV, false, 32>), grid, block, lds, st, P);
+#define HX_R16S(HH, RMV, SV)                                                                                \
+    if (P.inverse) hipLaunchKernelGGL((k_ntt_pass_r16<HH, true, false, RMV, SV>), grid, block, lds, st, P); \
+    else hipLaunchKernelGGL((k_ntt_pass_r16<HH, false, false, false, SV>), grid, block, lds, st, P);
+        if (P.rm.on && (!P.inverse || lm))
+            raise(HX_ERR_RUNTIME, "internal: accumulate epilogue on a forward or lane-major pass");
+        if (P.rm.on && S == 32) {
+            if (P.H == 8) {
+                HX_R16S(8, true, 32)
+            } else if (P.H == 7) {
+                HX_R16S(7, true, 32)
+            } else {
+                HX_R16S(6, true, 32)
+            }
+        } else if (P.rm.on) {
+            if (P.H == 8) {
+                HX_R16S(8, true, 16)
+            } else if (P.H == 7) {
+                HX_R16S(7, true, 16)
+            } else {
+                HX_R16S(6, true, 16)
+            }
+        } else if (S == 16) {
+            if (P.H == 8) {
+                HX_R16S(8, false, 16)
+            } else if (P.H == 7) {
+                HX_R16S(7, false, 16)
+            } else {
+                HX_R16S(6, false, 16)
+            }
         } else if (P.H == 8) {
             if (lm) {
                 HX_R16(8, true)
@@ -1472,6 +1498,7 @@ void launch_pass(PassArgs& P, hipStream_t st) {
                 HX_R16(6, false)
             }
         }
+#undef HX_R16S
 #undef HX_R16
     } else {
         const size_t lds = (size_t)S * (W + 1) * 8;
@@ -1483,16 +1510,19 @@ void launch_pass(PassArgs& P, hipStream_t st) {
 // Tile shape of a pass: col_tiles puts adjacent columns of one row in a tile
 // (layouts with the limb axis innermost), otherwise adjacent rows of one
 // column (limb-major layouts).  P.H, P.ncols and P.bot_count must be set.
-void configure_tile(PassArgs& P, bool col_tiles, int npass) {
+// r16_tile: sub-problems per workgroup of a radix-16 pass, 32 or (a tile of
+// adjacent columns only) 16.
+void configure_tile(PassArgs& P, bool col_tiles, int npass, int r16_tile = kR16Tile) {
     if (r16_window(P.H)) {
+        const int T = col_tiles && r16_tile == 16 ? 16 : kR16Tile;
         if (col_tiles) {
             int sc = 1;
-            while (sc < kR16Tile && sc < P.ncols) sc <<= 1;
+            while (sc < T && sc < P.ncols) sc <<= 1;
             P.S_cols = sc;
-            P.S_bot = kR16Tile / sc;
+            P.S_bot = T / sc;
         } else {
             P.S_cols = 1;
-            P.S_bot = kR16Tile;
+            P.S_bot = T;
         }
         return;
     }

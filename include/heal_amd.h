@@ -290,9 +290,10 @@ int hx_keyswitch_plan_create(hx_keyswitch_plan** out, const hx_ntt_plan* fwd, co
 int hx_keyswitch_key_bytes(const hx_keyswitch_plan* plan, int64_t batch, size_t* bytes);
 /* The tuned shape of a plan: low_bits = index bits transformed inside the
  * middle kernel (the rest in the first and last), limb_group = limbs per
- * middle-kernel workgroup.  Either pointer may be NULL.  No counterpart in the
- * reference. */
-int hx_keyswitch_plan_config(const hx_keyswitch_plan* plan, int32_t* low_bits, int32_t* limb_group);
+ * middle-kernel workgroup, tile = sub-problems (columns) per workgroup of the
+ * first and last kernels (32 or 16).  Any pointer may be NULL.  No counterpart
+ * in the reference. */
+int hx_keyswitch_plan_config(const hx_keyswitch_plan* plan, int32_t* low_bits, int32_t* limb_group, int32_t* tile);
 int hx_keyswitch_prepare_key(const hx_keyswitch_plan* plan, const hx_view* evk, void* key, hx_stream stream);
 int hx_keyswitch_exec(const hx_keyswitch_plan* plan, const hx_view* x, const void* key, size_t key_bytes,
                       const hx_view* out, hx_stream stream);
