@@ -86,8 +86,8 @@ __device__ __forceinline__ int ks_slot(int w) {
     return w + (w >> 4);
 }
 
-// The forward values of a MODE_L64 limb ([0, 8q)) enter the key inner product
-// half reduced, in [0, 4q), when the sum of D products stays below q * 2^64:
+// The forward values of a MODE_L64 limb ([0, 16q)) enter the key inner product
+// partly reduced, in [0, 4q), when the sum of D products stays below q * 2^64:
 // D * 4q * q <= q * 2^64.
 __device__ __forceinline__ bool ks_lazy_dot(const LimbInfo& L, int D) {
     return limb_mode(L) == MODE_L64 && (uint64_t)(4 * D) <= ~0ull / L.q;
@@ -226,10 +226,13 @@ __device__ __forceinline__ void ks_tile(const InnerArgs& A, int64_t blk, int64_t
             for (int e = 0; e < 16; ++e) x[e] = xs[ks_slot(r16_wB<H>(e, lane))];
             r16_stages_any<H, false, 1>(x, lane, twrow, hi, A.logm, A.hhi, L, mode);
             if (mode == MODE_L64) {
-                const uint64_t q4 = L.q << 2;
+                const uint64_t q8 = L.q << 3, q4 = L.q << 2;
                 if (ks_lazy_dot(L, D)) {
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) x[e] = x[e] >= q4 ? x[e] - q4 : x[e];
+                    for (int e = 0; e < 16; ++e) {
+                        const uint64_t v = x[e] >= q8 ? x[e] - q8 : x[e];
+                        x[e] = v >= q4 ? v - q4 : v;
+                    }
                 } else {
 #pragma unroll
                     for (int e = 0; e < 16; ++e) x[e] = lazy_reduce(x[e], L.q);

@@ -465,9 +465,12 @@ void ntt_plan_exec_noalloc(const hx_ntt_plan* plan, const hx_view* a, const hx_v
 // ------------------------------------------------------------ VALU ceiling
 // Throughput of the passes' 60-bit butterfly (MODE_L64, forward) with its
 // operands in registers: every thread runs 16 values as 8 independent
-// butterflies per round, each with its own twiddle pair, at 8 waves per SIMD.  No memory traffic but one store per thread at the end, so its rate
-// is the arithmetic ceiling of a pass of these butterflies on this GPU.
-constexpr int kRateRounds = 1024;
+// butterflies per stage, each with its own twiddle pair, two stages per round
+// (one that reduces its upper inputs and one that does not, as the passes
+// alternate them), at 8 waves per SIMD.  No memory traffic but one store per
+// thread at the end, so its rate is the arithmetic ceiling of a pass of these
+// butterflies on this GPU.
+constexpr int kRateRounds = 512;
 
 __global__ __launch_bounds__(256) void k_butterfly_rate(uint64_t* out, uint64_t q, uint64_t seed) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -482,7 +485,9 @@ __global__ __launch_bounds__(256) void k_butterfly_rate(uint64_t* out, uint64_t 
     }
     for (int r = 0; r < kRateRounds; ++r) {
 #pragma unroll
-        for (int i = 0; i < 16; i += 2) butterfly<MODE_L64, false>(x[i], x[i + 1], w[i / 2], wp[i / 2], q);
+        for (int i = 0; i < 16; i += 2) butterfly<MODE_L64, false, true>(x[i], x[i + 1], w[i / 2], wp[i / 2], q);
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) butterfly<MODE_L64, false, false>(x[i], x[i + 1], w[i / 2], wp[i / 2], q);
     }
     uint64_t h = 0;
 #pragma unroll
@@ -524,7 +529,7 @@ double butterfly_rate(hipStream_t st) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     check_hip(hipFreeAsync(out, st), "hipFreeAsync(rate)");
-    const double bflies = (double)blocks * threads * kRateRounds * 8;
+    const double bflies = (double)blocks * threads * kRateRounds * 16;
     return bflies / (best * 1e-3);
 }
 

@@ -238,7 +238,7 @@ struct PassArgs {
     RmacArgs rm;             // store epilogue of the key-switch's last pass (rm.on)
     int32_t reduce_in;
     int32_t scale_out;
-    int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 8q) forward, [0, 4q) inverse)
+    int32_t lazy_out;        // radix-16 passes: leave 64-bit residues lazy ([0, 16q) forward, [0, 4q) inverse)
     int32_t lane_major;      // radix-16 passes: a sub-problem's lanes are adjacent threads (rows contiguous in memory)
     int32_t map_bitrev;      // src_map / dst_map is the bit reversal of logm bits: rows are computed, not loaded
     int32_t top_bitrev;      // radix-16 pass over the low bits (b0 = 0) of rows-contiguous columns whose row map is
@@ -485,20 +485,26 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs P) {
 //
 // Arithmetic per limb (uniform over a sub-problem):
 //   MODE_S32  q < 2^31: 32-bit Shoup, residues kept in [0, q);
-//   MODE_L64  q < 2^61: Harvey's lazy butterflies (D. Harvey, "Faster
+//   MODE_L64  q < 2^60: Harvey's lazy butterflies (D. Harvey, "Faster
 //             arithmetic for number-theoretic transforms", J. Symb. Comp. 60,
-//             2014) on the three-product Shoup remainder in [0, 4q)
-//             (shoup_lazy3_nq), with every bound doubled: forward values stay
-//             in [0, 8q), inverse values in [0, 4q), reduced when leaving the
-//             transform;
+//             2014) on the three-product Shoup remainder t in [0, 4q)
+//             (shoup_lazy3_nq).  Forward: the upper input u is reduced (one
+//             conditional subtraction of 8q) only at the even stages of a
+//             window (RED), which keeps every value below 16q <= 2^64:
+//                 RED:  u < 16q -> u < 8q,  outputs u + t, u - t + 4q < 12q;
+//                 else: u < 12q,            outputs < 16q.
+//             A pass starts at an even stage, so any lazy input below 16q is
+//             valid.  Inverse values stay in [0, 4q).  Both are reduced when
+//             leaving the transform;
 //   MODE_F64  q < 2^63: 64-bit Shoup, residues kept in [0, q).
 enum : int { MODE_S32 = 0, MODE_L64 = 1, MODE_F64 = 2 };
 
 __device__ __forceinline__ int limb_mode(const LimbInfo& L) {
-    return L.small ? MODE_S32 : (L.q < (uint64_t(1) << 61) ? MODE_L64 : MODE_F64);
+    return L.small ? MODE_S32 : (L.q < (uint64_t(1) << 60) ? MODE_L64 : MODE_F64);
 }
 
-template <int MODE, bool INV>
+// RED: reduce the forward butterfly's upper input (MODE_L64; see above)
+template <int MODE, bool INV, bool RED = true>
 __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w, uint64_t wp, uint64_t q) {
     if (MODE == MODE_S32) {
         uint32_t q32 = (uint32_t)q, u = (uint32_t)x0, v = (uint32_t)x1;
@@ -514,9 +520,13 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
             x1 = shoup_mul32(d, (uint32_t)w, (uint32_t)wp, q32);
         }
     } else if (MODE == MODE_L64) {
-        const uint64_t q4 = q << 2;  // 8q <= 2^64
-        if (!INV) {  // x0, x1 in [0, 8q) -> [0, 8q)
-            uint64_t u = x0 >= q4 ? x0 - q4 : x0;                    // [0, 4q)
+        const uint64_t q4 = q << 2;  // 16q <= 2^64
+        if (!INV) {
+            uint64_t u = x0;
+            if (RED) {
+                const uint64_t q8 = q << 3;
+                u = x0 >= q8 ? x0 - q8 : x0;  // [0, 16q) -> [0, 8q)
+            }
             uint64_t t = shoup_lazy3_nq(x1, w, wp, (uint64_t)0 - q);  // [0, 4q)
             x0 = u + t;
             x1 = u - t + q4;
@@ -540,9 +550,10 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
     }
 }
 
-// Fully reduce a lazy value of a MODE_L64 limb (x < 8q).
+// Fully reduce a lazy value of a MODE_L64 limb (x < 16q).
 __device__ __forceinline__ uint64_t lazy_reduce(uint64_t x, uint64_t q) {
-    const uint64_t q4 = q << 2, q2 = q << 1;
+    const uint64_t q8 = q << 3, q4 = q << 2, q2 = q << 1;
+    x = x >= q8 ? x - q8 : x;
     x = x >= q4 ? x - q4 : x;
     x = x >= q2 ? x - q2 : x;
     return x >= q ? x - q : x;
@@ -654,11 +665,13 @@ __device__ __forceinline__ void r16_stages(uint64_t (&x)[16], int lane, const ul
                 butterfly_last_scaled<MODE>(x[e], x[e | (1 << eb)], L);
             }
         } else {
+            // the window's stage is k (layout A) or 4 + k (layout B): even for even k
             int i = 0;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 if (e & (1 << eb)) continue;
-                butterfly<MODE, INV>(x[e], x[e | (1 << eb)], cur[i].x, cur[i].y, q);
+                if (k & 1) butterfly<MODE, INV, false>(x[e], x[e | (1 << eb)], cur[i].x, cur[i].y, q);
+                else butterfly<MODE, INV, true>(x[e], x[e | (1 << eb)], cur[i].x, cur[i].y, q);
                 ++i;
             }
         }
