@@ -32,9 +32,9 @@ SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs at 2.4 GHz
 PROFILED = {
     (65536, 24, 16): {
         "ntt_pass_traffic": (1278601592, "profiles/ntt_pass_hbm_traffic.md"),
-        "ntt_pass_valu": (137170944, "profiles/ntt_pass_sq_counters.md"),
+        "ntt_pass_valu": (128507904, "profiles/ntt_pass_sq_counters.md"),
         "k2_traffic": (2427128704, "profiles/ntt_pass_hbm_traffic.md"),
-        "k2_valu": (258318336, "profiles/ntt_pass_sq_counters.md"),
+        "k2_valu": (252641280, "profiles/ntt_pass_sq_counters.md"),
     }
 }
 
