@@ -8,8 +8,7 @@
 // omega = psi^2, i = i1 + 256 i2 and u = u2 + 256 u1:
 //   step 1  C[u2, i1]  = sum_i2 W2[u2, i2] a[256 i2 + i1],  W2 = omega^(256 u2 i2) psi^(256 i2)
 //           C'[u2, i1] = C[u2, i1] * T[u2, i1] mod q,       T  = psi^i1 omega^(u2 i1)
-//           stored transposed: mid[i1][u2] = C'[u2, i1]
-//   step 2  out[256 u1 + u2] = sum_i1 W1[u1, i1] mid[i1][u2], W1 = omega^(256 u1 i1)
+//   step 2  out[256 u1 + u2] = sum_i1 W1[u1, i1] C'[u2, i1], W1 = omega^(256 u1 i1)
 // Both steps are 256 x 256 x 256 products per column with a constant left
 // matrix.  Operands (< 2^28) are split into four 7-bit int8 planes; the 16
 // plane products accumulate in int32 grouped by s = l + k (each group sums at
@@ -23,6 +22,13 @@
 // A and column l mod 16 of B with 16 int8 each, k = 16 (l / 16) + i within a
 // 64-wide k chunk (A and B agree); the 16 x 16 int32 result gives lane l rows
 // 4 (l / 16) + j, j < 4, of column l mod 16.
+//
+// Step 1 computes the transpose C^T = a^T W2^T (the data planes as A, the W2
+// planes as B: the two operand layouts agree, so the same registers serve).
+// A lane then holds C'[u2, i1] for four consecutive i1 of one u2, which are the
+// four bytes of one word of step 2's right operand (k = i1, column u2), and
+// step 1 stores C' as step 2's int8 planes in step 2's LDS operand order
+// (`mid` below); step 2 copies its operand block with 16-byte loads.
 #include <cstring>
 #include <new>
 
@@ -55,7 +61,7 @@ struct Ntt4Args {
     int64_t a_sp, a_st, a_si;
     int64_t* out;
     int64_t o_sp, o_st, o_si;
-    uint32_t* mid;         // [P*K][256 i1][256 u2]
+    uint32_t* mid;         // step 2's right operand planes: per column [8 c-blocks][2 c-tiles][4 planes][4 k-chunks][64 lanes] x 16 bytes
     const uint4* w;        // the step's left matrix planes
     const uint32_t* tmat;
     const uint64_t* red;
@@ -95,22 +101,17 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
     const uint64_t q = rd[0], mu = rd[1];
     const int tid = threadIdx.x;
 
-    // ---- right operand: 256 k rows x 32 columns, split into planes
-    {
+    // ---- the data operand: 256 k rows x 32 columns as int8 planes in LDS
+    if (STEP == 1) {
+        // split here: a[256 i2 + i1], k = i2, column i1
         const int c = tid % kBlockCols, run = tid / kBlockCols;  // 16 runs of 16 k
         const int gc = cb * kBlockCols + c;
         uint32_t v[16];
-        if (STEP == 1) {
-            const int64_t* base = A.a + p * A.a_sp + t * A.a_st;
+        const int64_t* base = A.a + p * A.a_sp + t * A.a_st;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int64_t k = 16 * run + i;  // i2
-                v[i] = (uint32_t)reduce_in(base[(k * kRows + gc) * A.a_si], q);
-            }
-        } else {
-            const uint32_t* base = A.mid + col * (kRows * kRows);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) v[i] = base[(16 * run + i) * kRows + gc];  // mid[i1][u2]
+        for (int i = 0; i < 16; ++i) {
+            const int64_t k = 16 * run + i;  // i2
+            v[i] = (uint32_t)reduce_in(base[(k * kRows + gc) * A.a_si], q);
         }
         const int ct = c / 16, lane = (run % 4) * 16 + c % 16, kq = run / 4;
 #pragma unroll
@@ -125,6 +126,11 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
             }
             bs[((ct * 4 + l) * 4 + kq) * 64 + lane] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
         }
+    } else {
+        // step 1 stored this column block's planes in this LDS order
+        const uint4* src = (const uint4*)A.mid + (col * kColBlocks + cb) * (2 * 4 * 4 * 64);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bs[tid + i * kThreads4] = src[tid + i * kThreads4];
     }
     __syncthreads();
 
@@ -169,7 +175,8 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
             for (int la = 0; la < 4; ++la)
 #pragma unroll
                 for (int lb = 0; lb < 4; ++lb)
-                    acc[ct][la + lb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[la], bm[ct][lb], acc[ct][la + lb], 0, 0, 0);
+                    acc[ct][la + lb] = STEP == 1 ? __builtin_amdgcn_mfma_i32_16x16x64_i8(bm[ct][lb], am[la], acc[ct][la + lb], 0, 0, 0)
+                                                 : __builtin_amdgcn_mfma_i32_16x16x64_i8(am[la], bm[ct][lb], acc[ct][la + lb], 0, 0, 0);
         if (kq + 1 < 4) {
 #pragma unroll
             for (int l = 0; l < 4; ++l) am[l] = an[l];
@@ -181,7 +188,39 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
     uint64_t R[7];
 #pragma unroll
     for (int s = 0; s < 7; ++s) R[s] = rd[2 + s];
-    {
+    if (STEP == 1) {
+        // lane: C'[u2, i1] for u2 = ut*16 + lane%16 and i1 = i0 + j, j < 4, of each c-tile
+        const int u2 = ut * 16 + lane % 16;
+        const uint32_t* trow = A.tmat + (t * kRows + u2) * kRows;  // T[u2][*]
+        uint32_t* planes = A.mid + (int64_t)col * (kRows * kRows);
+        // step 2's operand entry of (u2, i1): c-block u2/32, c-tile (u2/16)%2, k-chunk i1/64,
+        // lane ((i1%64)/16)*16 + u2%16, word (i1%16)/4, byte i1%4
+        const int cb2 = u2 >> 5, ct2 = (u2 >> 4) & 1, kq2 = cb >> 1;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int i0 = cb * kBlockCols + ct * 16 + 4 * (lane / 16);
+            const uint4 tw4 = *(const uint4*)(trow + i0);
+            const uint32_t tws[4] = {tw4.x, tw4.y, tw4.z, tw4.w};
+            uint32_t r4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint64_t v = 0;
+#pragma unroll
+                for (int s = 0; s < 7; ++s) v += (uint64_t)(uint32_t)acc[ct][s][j] * R[s];
+                v = barrett(v, q, mu);
+                r4[j] = (uint32_t)barrett(v * tws[j], q, mu);
+            }
+            const int lane2 = (((cb & 1) * 2 + ct) << 4) + lane % 16;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) word |= ((r4[j] >> (7 * l)) & 127u) << (8 * j);
+                const int64_t e = ((((int64_t)cb2 * 2 + ct2) * 4 + l) * 4 + kq2) * 64 + lane2;  // 16-byte entry
+                planes[e * 4 + lane / 16] = word;
+            }
+        }
+    } else {
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
             const int row0 = ut * 16 + 4 * (lane / 16);
@@ -192,21 +231,12 @@ __global__ __launch_bounds__(kThreads4) void k_ntt4(Ntt4Args A) {
                 uint64_t v = 0;
 #pragma unroll
                 for (int s = 0; s < 7; ++s) v += (uint64_t)(uint32_t)acc[ct][s][j] * R[s];
-                v = barrett(v, q, mu);
-                if (STEP == 1) {
-                    const uint64_t tw = A.tmat[(t * kRows + row0 + j) * kRows + gc];  // T[u2][i1]
-                    v = barrett(v * tw, q, mu);
-                }
-                r4[j] = (uint32_t)v;
+                r4[j] = (uint32_t)barrett(v, q, mu);
             }
-            if (STEP == 1) {  // mid[i1 = gc][u2 = row0 .. row0 + 3]
-                *(uint4*)(A.mid + col * (kRows * kRows) + (int64_t)gc * kRows + row0) =
-                    make_uint4(r4[0], r4[1], r4[2], r4[3]);
-            } else {          // out[256 u1 + u2], u1 = row, u2 = gc
-                int64_t* ob = A.out + p * A.o_sp + t * A.o_st;
+            // out[256 u1 + u2], u1 = row, u2 = gc
+            int64_t* ob = A.out + p * A.o_sp + t * A.o_st;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) ob[((int64_t)(row0 + j) * kRows + gc) * A.o_si] = (int64_t)r4[j];
-            }
+            for (int j = 0; j < 4; ++j) ob[((int64_t)(row0 + j) * kRows + gc) * A.o_si] = (int64_t)r4[j];
         }
     }
 }
