@@ -190,6 +190,45 @@ HX_HD uint64_t shoup_lazy3_nq(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) 
     return ((uint64_t)hi << 32) | (uint32_t)p;
 }
 
+// Lazy radix-2 butterflies for 64-bit moduli q < 2^60 (D. Harvey, "Faster
+// arithmetic for number-theoretic transforms", J. Symb. Comp. 60, 2014) on the
+// three-product remainder t = shoup_lazy3_nq(x1, w, wp, -q) in [0, 4q):
+//   forward (Cooley-Tukey)  x0 <- u + t, x1 <- u - t + 4q, u = x0 reduced by a
+//       conditional subtraction of 8q when RED.  RED: x0 < 16q gives u < 8q and
+//       outputs below 12q; without RED x0 < 12q gives outputs below 16q <= 2^64.
+//       Alternating the two (a reducing stage first) keeps every value below 16q.
+//   inverse (Gentleman-Sande)  x0 <- (x0 + x1) reduced below 4q, x1 <- (x0 - x1 +
+//       4q) w, for x0, x1 < 4q.
+template <bool RED>
+HX_HD void lazy_fwd_bfly(uint64_t& x0, uint64_t& x1, uint64_t w, uint64_t wp, uint64_t q) {
+    uint64_t u = x0;
+    if (RED) {
+        const uint64_t q8 = q << 3;
+        u = x0 >= q8 ? x0 - q8 : x0;
+    }
+    const uint64_t t = shoup_lazy3_nq(x1, w, wp, (uint64_t)0 - q);
+    const uint64_t q4 = q << 2;
+    x0 = u + t;
+    x1 = u - t + q4;
+}
+
+HX_HD void lazy_inv_bfly(uint64_t& x0, uint64_t& x1, uint64_t w, uint64_t wp, uint64_t q) {
+    const uint64_t q4 = q << 2;
+    const uint64_t a = x0 + x1;
+    const uint64_t d = x0 - x1 + q4;
+    x0 = a >= q4 ? a - q4 : a;
+    x1 = shoup_lazy3_nq(d, w, wp, (uint64_t)0 - q);
+}
+
+// The residue in [0, q) of a lazy value x < 16q.
+HX_HD uint64_t lazy_reduce(uint64_t x, uint64_t q) {
+    const uint64_t q8 = q << 3, q4 = q << 2, q2 = q << 1;
+    x = x >= q8 ? x - q8 : x;
+    x = x >= q4 ? x - q4 : x;
+    x = x >= q2 ? x - q2 : x;
+    return x >= q ? x - q : x;
+}
+
 // Montgomery reduction with R = 2^64 (P. L. Montgomery, "Modular multiplication
 // without trial division", Math. Comp. 44, 1985).  For odd q: qni = -q^-1 mod
 // 2^64 by Newton's iteration x <- x (2 - q x), each step doubling the correct

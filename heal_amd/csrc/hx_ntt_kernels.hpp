@@ -520,22 +520,8 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
             x1 = shoup_mul32(d, (uint32_t)w, (uint32_t)wp, q32);
         }
     } else if (MODE == MODE_L64) {
-        const uint64_t q4 = q << 2;  // 16q <= 2^64
-        if (!INV) {
-            uint64_t u = x0;
-            if (RED) {
-                const uint64_t q8 = q << 3;
-                u = x0 >= q8 ? x0 - q8 : x0;  // [0, 16q) -> [0, 8q)
-            }
-            uint64_t t = shoup_lazy3_nq(x1, w, wp, (uint64_t)0 - q);  // [0, 4q)
-            x0 = u + t;
-            x1 = u - t + q4;
-        } else {  // x0, x1 in [0, 4q) -> [0, 4q)
-            uint64_t a = x0 + x1;
-            uint64_t d = x0 - x1 + q4;
-            x0 = a >= q4 ? a - q4 : a;
-            x1 = shoup_lazy3_nq(d, w, wp, (uint64_t)0 - q);
-        }
+        if (!INV) lazy_fwd_bfly<RED>(x0, x1, w, wp, q);
+        else lazy_inv_bfly(x0, x1, w, wp, q);
     } else {
         if (!INV) {
             uint64_t t = shoup_mul(x1, w, wp, q);
@@ -550,14 +536,6 @@ __device__ __forceinline__ void butterfly(uint64_t& x0, uint64_t& x1, uint64_t w
     }
 }
 
-// Fully reduce a lazy value of a MODE_L64 limb (x < 16q).
-__device__ __forceinline__ uint64_t lazy_reduce(uint64_t x, uint64_t q) {
-    const uint64_t q8 = q << 3, q4 = q << 2, q2 = q << 1;
-    x = x >= q8 ? x - q8 : x;
-    x = x >= q4 ? x - q4 : x;
-    x = x >= q2 ? x - q2 : x;
-    return x >= q ? x - q : x;
-}
 
 // ---------------------------------------------------------------------------
 // Register-radix-16 pass for windows of H = 6..8 bits (the NTT passes).

@@ -115,6 +115,72 @@ int main() {
             CHECK(hx::cpp_rem(mi, sa) == (int64_t)er, "cpp_rem");
         }
     }
+    // 60-bit lazy butterflies (q < 2^60): bounds of each stage kind at the extremes
+    // of their input ranges, congruence with exact arithmetic, and chains of
+    // alternating stages (a reducing stage first, as every pass starts)
+    for (int it = 0; it < 20000; ++it) {
+        uint64_t q = (it % 4 == 0) ? ((1ull << 60) - 1 - 2 * (rng() % 4096)) : ((rng() >> 4) | 1);
+        if (q < 3) q = 3;
+        const hx::ModInfo mi = hx::make_modinfo(q);
+        const uint64_t w = (it % 7 == 0) ? q - 1 : rng() % q;
+        const uint64_t wp = hx::shoup_companion(mi, w);
+        const uint64_t x1s[4] = {rng(), ~0ull, rng() % (16 * q), 0};
+        auto mulw = [&](uint64_t x) { return (uint64_t)(((u128)(x % q) * w) % q); };
+        // reducing stage: x0 < 16q -> outputs < 12q
+        const uint64_t r0s[5] = {16 * q - 1, 8 * q, 8 * q - 1, 0, rng() % (16 * q)};
+        for (uint64_t a : r0s)
+            for (uint64_t b : x1s) {
+                uint64_t x0 = a, x1 = b;
+                hx::lazy_fwd_bfly<true>(x0, x1, w, wp, q);
+                CHECK(x0 < 12 * q && x1 < 12 * q, "fwd RED bound q=%llu", (unsigned long long)q);
+                CHECK(x0 % q == (a % q + mulw(b)) % q && x1 % q == (a % q + q - mulw(b)) % q, "fwd RED value");
+            }
+        // non-reducing stage: x0 < 12q -> outputs < 16q
+        const uint64_t n0s[4] = {12 * q - 1, 0, rng() % (12 * q), 4 * q};
+        for (uint64_t a : n0s)
+            for (uint64_t b : x1s) {
+                uint64_t x0 = a, x1 = b;
+                hx::lazy_fwd_bfly<false>(x0, x1, w, wp, q);
+                CHECK(x0 < 16 * q && x1 < 16 * q, "fwd bound q=%llu", (unsigned long long)q);
+                CHECK(x0 % q == (a % q + mulw(b)) % q && x1 % q == (a % q + q - mulw(b)) % q, "fwd value");
+            }
+        // inverse: x0, x1 < 4q -> both < 4q
+        const uint64_t i0s[3] = {4 * q - 1, 0, rng() % (4 * q)};
+        for (uint64_t a : i0s)
+            for (uint64_t b : i0s) {
+                uint64_t x0 = a, x1 = b;
+                hx::lazy_inv_bfly(x0, x1, w, wp, q);
+                CHECK(x0 < 4 * q && x1 < 4 * q, "inv bound q=%llu", (unsigned long long)q);
+                CHECK(x0 % q == (a + b) % q && x1 % q == mulw((a % q + q - b % q) % q), "inv value");
+            }
+        const uint64_t lz[4] = {16 * q - 1, 8 * q, rng() % (16 * q), q};
+        for (uint64_t a : lz) CHECK(hx::lazy_reduce(a, q) == a % q, "lazy_reduce q=%llu", (unsigned long long)q);
+        // a 16-point network of 4 stages twice over (8 stages, pairs i, i ^ 2^(s mod 4)), each
+        // stage's twiddles random; residues tracked exactly alongside
+        if (it % 8 == 0) {
+            uint64_t x[16], e[16];
+            for (int i = 0; i < 16; ++i) {
+                x[i] = (i == 0) ? 16 * q - 1 : rng() % (16 * q);
+                e[i] = x[i] % q;
+            }
+            for (int st = 0; st < 8; ++st) {
+                const int h = 1 << (3 - st % 4);
+                const uint64_t ws = rng() % q, wsp = hx::shoup_companion(mi, ws);
+                for (int i = 0; i < 16; ++i) {
+                    if (i & h) continue;
+                    if (st % 2 == 0) hx::lazy_fwd_bfly<true>(x[i], x[i + h], ws, wsp, q);
+                    else hx::lazy_fwd_bfly<false>(x[i], x[i + h], ws, wsp, q);
+                    const uint64_t t = (uint64_t)(((u128)e[i + h] * ws) % q);
+                    const uint64_t u = e[i];
+                    e[i] = (u + t) % q;
+                    e[i + h] = (u + q - t) % q;
+                }
+                for (int i = 0; i < 16; ++i)
+                    CHECK(x[i] < (st % 2 == 0 ? 12 * q : 16 * q) && x[i] % q == e[i], "chain stage %d q=%llu", st,
+                          (unsigned long long)q);
+            }
+        }
+    }
     if (failures) {
         std::printf("FAILED %d of %ld\n", failures, checks);
         return 1;
