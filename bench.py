@@ -15,28 +15,52 @@ of the same shape and checks its output against the GPU's.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
+import re
 import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs at 2.4 GHz
-# Profiled constants of the current kernels on the bench workload, keyed by
-# (N, L, B); each entry names the profile it comes from and goes stale when the
-# kernel changes.  HBM bytes per launch: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (one pass each), scaled by the same counter's reading for a
-# 256 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1).  VALU instructions per
-# launch: SQ_INSTS_VALU.  The NTT pass figures average its two launches.
+# Profiled constants of the bench workload's kernels, keyed by (N, L, B); each
+# entry names the profile it comes from.  HBM bytes per launch: rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (one pass each), scaled by the same counter's
+# reading for a plain device copy (FETCH_SIZE x 2, WRITE_SIZE x 1).  VALU
+# instructions per launch: SQ_INSTS_VALU.  The NTT pass figures average its two
+# launches.  "kernels" is the fingerprint of the kernel sources they were
+# measured on (kernel_fingerprint()); on other sources the bench reports them
+# as stale and leaves traffic and instruction counts null.
 PROFILED = {
     (65536, 24, 16): {
-        "ntt_pass_traffic": (1278601592, "profiles/ntt_pass_hbm_traffic.md"),
+        "kernels": "f979ed5be43d8b54",
+        "ntt_pass_traffic": (1298226060, "profiles/ntt_pass_hbm_traffic.md"),
         "ntt_pass_valu": (128507904, "profiles/ntt_pass_sq_counters.md"),
-        "k2_traffic": (2427128704, "profiles/ntt_pass_hbm_traffic.md"),
+        "k2_traffic": (2427252794, "profiles/ntt_pass_hbm_traffic.md"),
         "k2_valu": (252641280, "profiles/ntt_pass_sq_counters.md"),
     }
 }
+KERNEL_SOURCES = ("heal_amd/csrc/hx_arith.hpp", "heal_amd/csrc/hx_internal.hpp", "heal_amd/csrc/hx_ntt_kernels.hpp",
+                  "heal_amd/csrc/hx_ntt.hip", "heal_amd/csrc/hx_keyswitch.hip")
+
+
+def source_code_only(text: str) -> str:
+    """C/C++ source without comments, whitespace runs collapsed."""
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return re.sub(r"\s+", " ", text).strip()
+
+
+def kernel_fingerprint() -> str:
+    """SHA-256 (first 16 hex digits) of the kernel sources' code: a comment or
+    layout edit keeps it, a code change does not."""
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(HERE, rel)) as f:
+            h.update(source_code_only(f.read()).encode())
+    return h.hexdigest()[:16]
 
 
 def parse():
@@ -203,7 +227,6 @@ def main():
     avg_launch = t_ntt / (reps * n_launch)
     achieved = bytes_per_launch / avg_launch / 1e9
     ntt_limb_gbs = ntt_bytes * reps / t_ntt / 1e9
-    traffic = PROFILED.get((N, L, B), {}).get("ntt_pass_traffic", (None,))[0]
     limbs_inv = B * 2 * L
     t_intt = timed(lambda: batched.inv(batched.acc.view(B * 2, N, 1, L), batched.coef), reps, torch)
     intt_limb_gbs = limbs_inv * 2 * N * 8 * reps / t_intt / 1e9
@@ -235,10 +258,16 @@ def main():
     assert torch.equal(fused_out, ref_all), "the timed fused runs changed their output"
 
     prof = PROFILED.get((N, L, B), {})
+    fingerprint = kernel_fingerprint()
+    prof_current = prof.get("kernels") == fingerprint
 
     def profiled(key):
         v = prof.get(key)
-        return (v[0], v[1]) if v else (None, None)
+        if not v:
+            return None, None
+        if not prof_current:
+            return None, f"{v[1]} (measured on kernels {prof.get('kernels')}, these are {fingerprint}: stale, {v[0]})"
+        return v[0], v[1]
 
     # K2: forward low window of the digit columns, key inner product, inverse
     # low window.  Its bytes as a kernel: the D digit-NTT intermediate columns
@@ -316,7 +345,7 @@ def main():
                               "kernel": f"k_ntt_pass_r16<8,false>: forward NTT of {limbs_fwd} limbs of N={N}, "
                                         f"{n_launch} launches per NTT",
                               "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled("ntt_pass_traffic")[0],
                               "traffic_source": profiled("ntt_pass_traffic")[1],
                               "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_launch * 1e3, 4),
                               "launch_read_write_GBps": round(2 * achieved, 1),

@@ -108,3 +108,21 @@ def test_capi_errors_without_gpu_work():
         capi.check(capi.lib.hx_modmul_ttt(None, None, None, None, None))
     with pytest.raises(ValueError):
         capi.check(capi.lib.hx_mod_tt(None, None, None, None))
+
+
+def test_bench_kernel_fingerprint_tracks_code_not_comments():
+    """bench.py's profiled traffic and instruction counts carry the fingerprint
+    of the kernel sources they were measured on: comment and layout edits keep
+    it, a code change does not (the bench then reports the constants as stale)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    code = "int f(int x) {\n    return x + 1;  // one more\n}\n"
+    same = "/* doc */ int f(int x)\n{ return x + 1; }\n// trailing note\n"
+    other = "int f(int x) {\n    return x + 2;\n}\n"
+    assert bench.source_code_only(code) == bench.source_code_only(same)
+    assert bench.source_code_only(code) != bench.source_code_only(other)
+    fp = bench.kernel_fingerprint()
+    assert len(fp) == 16 and all(len(e["kernels"]) == 16 for e in bench.PROFILED.values())
