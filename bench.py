@@ -190,6 +190,16 @@ def main():
 
     t_graph = timed(fused_graph.replay, args.steps, torch)
     t_fused = timed(lambda: batched.run_fused(fused_out), args.steps, torch)
+    # beside it: the batch with one evaluation key for every instance, as the
+    # serving loop applies one rotation key to a batch (K2 then reads each key
+    # block once per limb group, from L2 for the other instances); instance 0
+    # has the same x and key as in the headline form
+    key1 = batched.fused.prepare_key(batched.evk[:, :1].contiguous())
+    shared_out = torch.empty_like(fused_out)
+    t_shared = timed(lambda: batched.fused.exec_ex(batched.x, key1, shared_out, key_limbs=L, key_count=1),
+                     args.steps, torch)
+    torch.cuda.synchronize()
+    assert torch.equal(shared_out[0], fused_out[0]), "shared-key form differs on instance 0"
     # the same K steps again with events between the three kernels (HIP events
     # on the launch stream), for the per-kernel times of the dominant kernel
     marks = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
@@ -321,6 +331,7 @@ def main():
                    "batched_chain_eager_ks_per_s": round(ks_batched, 2),
                    "batched_chain_graph_ks_per_s": round(ks_chain_graph, 2),
                    "fused_graph_ks_per_s": round(ks_graph, 2),
+                   "fused_shared_key_ks_per_s": round(args.steps * B / t_shared, 2),
                    "fused_GBps_at_kernel_boundaries": round(fused_gbs, 1),
                    "ntt_limb_GBps": round(ntt_limb_gbs, 1), "intt_limb_GBps": round(intt_limb_gbs, 1),
                    "chain_call_hbm": elem_rates,

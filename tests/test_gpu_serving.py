@@ -196,6 +196,17 @@ def test_keyswitch_rmac_equals_keyswitch_then_rot_mac(gpu, B, n, L):
         assert torch.equal(got, want), r
     with pytest.raises(ValueError):  # acc must not overlap ct
         ks.exec_rmac(ct[:, 1], key, ct, ct, diags, 3, key_limbs=L, key_count=1, row_map=rmap)
+    acc = acc0.clone()
+    with pytest.raises(ValueError):  # M must be a power of two
+        ks.exec_rmac(ct[:, 1], key, acc, ct, diags[:, :96, :96].contiguous(), 3, key_limbs=L, key_count=1,
+                     row_map=rmap)
+    with pytest.raises(ValueError):  # diagonals are numbered from 0
+        ks.exec_rmac(ct[:, 1], key, acc, ct, diags, -1, key_limbs=L, key_count=1, row_map=rmap)
+    with pytest.raises(ValueError):  # ct must have acc's strides
+        ks.exec_rmac(ct[:, 1], key, acc, ct.transpose(0, 1).contiguous().transpose(0, 1), diags, 3, key_limbs=L,
+                     key_count=1, row_map=rmap)
+    torch.cuda.synchronize()
+    assert torch.equal(acc, acc0)  # refused calls leave acc alone
     ks.close()
     fwd.close()
     inv.close()
