@@ -4,7 +4,7 @@ fused key-switch plans under the tuning cache named by HEAL_AMD_TUNE_CACHE
 per check.
 
 mode "tune":  one plan at N = 4096, L = 3, dnum = 3; prints its configuration.
-mode "check": plans at N = 4096 and N = 65536; prints their configurations and
+mode "check": plans at N = 4096, 65536 and 32768; prints their configurations and
               whether the fused key-switch equals the six-call chain, whether
               hx_keyswitch_exec_rmac equals hx_keyswitch_exec_ex followed by
               hx_rot_mac, and whether a group's members (32-column tiles)
@@ -91,8 +91,9 @@ def group_vs_members(seed):
 
 
 def check():
-    for res in (fused_vs_chain(4096, 3, 2, 3), fused_vs_chain(65536, 24, 2, 5), rmac_vs_separate(4096, 4, 3, 21),
-                rmac_vs_separate(65536, 24, 2, 22), group_vs_members(7)):
+    for res in (fused_vs_chain(4096, 3, 2, 3), fused_vs_chain(65536, 24, 2, 5), fused_vs_chain(32768, 16, 2, 6),
+                rmac_vs_separate(4096, 4, 3, 21), rmac_vs_separate(65536, 24, 2, 22),
+                rmac_vs_separate(32768, 16, 2, 23), group_vs_members(7)):
         print(json.dumps(res), flush=True)
 
 

@@ -4,7 +4,8 @@ once per (GPU architecture, N, L, dnum), appends them to the cache file
 (HEAL_AMD_TUNE_CACHE) and the next process reads them back without timing.
 Plans whose cached tile is 16 compute what the six-call chain computes, alone,
 with the serving accumulate fused into their last kernel, and as members of a
-group (whose kernels use 32-column tiles).  Each case runs in a child process
+group (whose kernels use 32-column tiles), at first/last-kernel windows of 6, 7
+and 8 bits.  Each case runs in a child process
 (keyswitch_tile_child.py): the cache file is read once per process."""
 from __future__ import annotations
 
@@ -57,13 +58,16 @@ def test_plans_with_sixteen_column_tiles(gpu, tmp_path):
     rows = [f"{arch} 8192 4 3 6 1"]
     rows += [f"{arch} 4096 {k} 3 6 1 16" for k in range(1, 9)]
     rows.append(f"{arch} 65536 24 3 8 4 16")
+    rows.append(f"{arch} 32768 16 3 8 4 16")  # 8 low bits: K1/K3 windows of 7 bits
     cache.write_text("\n".join(rows) + "\n")
     results = _child(cache, "check")
-    assert [r["check"] for r in results] == ["fused", "fused", "rmac", "rmac", "group"]
+    assert [r["check"] for r in results] == ["fused", "fused", "fused", "rmac", "rmac", "rmac", "group"]
     for r in results:
         assert r["equal"], r
         if r["check"] == "group":
             assert r["tiles"] == [16], r
         else:
             assert r["config"]["tile"] == 16, r
+            if r["n"] == 32768:
+                assert r["config"]["low_bits"] == 8, r
     assert len(_lines(cache)) == len(rows)  # every plan found its shape in the file

@@ -157,13 +157,13 @@ def test_rot_mac_equals_the_five_calls(gpu):
         capi.rot_mac(got, dev["ks"], dev["ct"], None, dev["diags"], diag, dev["q"])
 
 
-@pytest.mark.parametrize("B,n,L", [(3, 4096, 4), (2, 65536, 24)])
+@pytest.mark.parametrize("B,n,L", [(3, 4096, 4), (2, 32768, 16), (2, 65536, 24)])
 def test_keyswitch_rmac_equals_keyswitch_then_rot_mac(gpu, B, n, L):
     """hx_keyswitch_exec_rmac (the rotate-multiply-accumulate in the key-switch's
     last kernel) against hx_keyswitch_exec_ex followed by hx_rot_mac, with
     signed and unreduced accumulators and diagonals; shared key, per-call row
-    map, two rotations of one plan.  The two ring sizes run the last pass with
-    64- and 256-point windows."""
+    map, two rotations of one plan, at three ring sizes (the last pass's window
+    is 6 bits at N = 2^12, 8 at N = 2^16, 7 or 8 at N = 2^15 as tuned)."""
     from heal_amd import capi
     from heal_amd import params as P
 
