@@ -1047,11 +1047,13 @@ static void keyswitch_tune(hx_keyswitch_plan* plan, hipStream_t st) {
             if (t < best) best = t, G = g;
         }
         // K1/K3 workgroups of 16 sub-problems (twice as many workgroups, each
-        // half the size) against the 32 timed above
+        // half the size) against the 32 timed above: taken only when clearly
+        // faster, since the two time within noise of each other where both
+        // fill the GPU and 32 is faster in the accumulate-fused last kernel
         if (G >= 0) {
             plan->G = G;
             plan->tile = 16;
-            if (time_keyswitch(plan, &xv, key, kb, &ov, ws, st, e0, e1) < best) tile = 16;
+            if (time_keyswitch(plan, &xv, key, kb, &ov, ws, st, e0, e1) < 0.97f * best) tile = 16;
             plan->tile = 32;
         }
     } catch (...) {
