@@ -35,10 +35,10 @@ SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs at 2.4 GHz
 # as stale and leaves traffic and instruction counts null.
 PROFILED = {
     (65536, 24, 16): {
-        "kernels": "f979ed5be43d8b54",
-        "ntt_pass_traffic": (1298226060, "profiles/ntt_pass_hbm_traffic.md"),
+        "kernels": "600ddb1e33776d1b",
+        "ntt_pass_traffic": (1298222348, "profiles/ntt_pass_hbm_traffic.md"),
         "ntt_pass_valu": (128507904, "profiles/ntt_pass_sq_counters.md"),
-        "k2_traffic": (2427252794, "profiles/ntt_pass_hbm_traffic.md"),
+        "k2_traffic": (2427036070, "profiles/ntt_pass_hbm_traffic.md"),
         "k2_valu": (252641280, "profiles/ntt_pass_sq_counters.md"),
     }
 }
