@@ -165,6 +165,33 @@ def test_fused_keyswitch_full_size_matches_plain_chain(gpu):
     ks.close()
 
 
+@pytest.mark.parametrize("n,limbs", [(4096, 4), (65536, 24)])
+def test_fused_keyswitch_extreme_residues(gpu, n, limbs):
+    """Every word of x and of the evaluation key q - 1 (w = 20: all three digits
+    live), so that the fused kernels' lazy values reach their bounds (K1's and
+    K2's forward stages below 16q, the reduction before the key product, the
+    inverse stages below 4q).  Fused equals the six-call chain; at N = 4096
+    instance 0 equals the oracle's chain."""
+    from heal_amd.keyswitch import KeySwitchBatch
+
+    print(f"[fused extreme] n={n} L={limbs}", flush=True)
+    ks = KeySwitchBatch(n=n, limbs=limbs, dnum=3, base_bits=20, batch=2, seed=13)
+    qm1 = ks.prm.p.to(torch.int64) - 1
+    ks.x.copy_(qm1.expand_as(ks.x))
+    ks.evk.copy_(qm1.expand_as(ks.evk))
+    ks.run()
+    torch.cuda.synchronize()
+    want = ks.out.clone()
+    out = torch.zeros_like(want)
+    ks.run_fused(out)
+    torch.cuda.synchronize()
+    assert ks.fused is not None  # the fused kernels took these inputs
+    assert torch.equal(out, want)
+    if n == 4096:
+        assert torch.equal(out[0].cpu(), oracle_keyswitch(ks, 0))
+    ks.close()
+
+
 def test_fused_keyswitch_errors(gpu):
     from heal_amd import capi, params as P
 
