@@ -269,23 +269,36 @@ def test_ntt_intt_parity(gpu, m, bits, dtype, layout, perm_kind, table):
         assert torch.equal(ig.cpu(), a)  # round trip
 
 
+EXTREME_CLASSES = [
+    # bits, dtype: the lazy 60-bit class at its largest moduli, the exact 64-bit class, 32-bit Shoup
+    ([60, 60], torch.int64),
+    ([62, 61], torch.int64),
+    ([30, 29], torch.int32),
+]
+
+
 @pytest.mark.parametrize("m", [4096, 65536])
 @pytest.mark.parametrize("pattern", ["max", "alternate"])
-def test_ntt_intt_extreme_residues_at_the_lazy_bound(gpu, m, pattern):
-    """Every input q - 1 (or q - 1 and 0 alternating) at the two largest primes
-    below 2^60, the largest moduli of the 60-bit lazy butterflies, with the psi
-    tables and with every twiddle q - 1: intermediates reach the lazy bounds
-    (below 16q forward, 4q inverse).  Forward and inverse equal the oracle's."""
-    primes = P.ntt_primes(m, [60, 60])
-    assert all((1 << 59) < q < (1 << 60) for q in primes)
+@pytest.mark.parametrize("cls", range(len(EXTREME_CLASSES)), ids=["lazy60", "exact64", "s32"])
+def test_ntt_intt_extreme_residues_at_the_lazy_bound(gpu, m, pattern, cls):
+    """Every input q - 1 (or q - 1 and 0 alternating), with the psi tables and
+    with every twiddle q - 1, in each butterfly class: at the two largest primes
+    below 2^60 the lazy 60-bit butterflies' intermediates reach their bounds
+    (below 16q forward, 4q inverse); 61/62-bit primes take the exact 64-bit
+    butterflies, 29/30-bit int32 the 32-bit ones.  Forward and inverse equal the
+    oracle's."""
+    bits, dtype = EXTREME_CLASSES[cls]
+    primes = P.ntt_primes(m, bits)
+    if cls == 0:
+        assert all((1 << 59) < q < (1 << 60) for q in primes)
     O, G = OracleBackend(), GpuBackend()
-    prm = P.NttParams(m, primes, dtype=torch.int64)
-    qv = torch.tensor(primes, dtype=torch.int64)
+    prm = P.NttParams(m, primes, dtype=dtype)
+    qv = torch.tensor(primes, dtype=dtype)
     a = (qv - 1).view(1, 1, 1, -1).expand(2, m, 1, len(primes)).contiguous()
     if pattern == "alternate":
         a[:, 1::2] = 0
     for table in ("psi", "max"):
-        print(f"[ntt extreme] m={m} {pattern} {table}", flush=True)
+        print(f"[ntt extreme] m={m} {bits} {pattern} {table}", flush=True)
         if table == "psi":
             tw, itw = prm.twiddles, prm.inv_twiddles
         else:
@@ -293,11 +306,11 @@ def test_ntt_intt_extreme_residues_at_the_lazy_bound(gpu, m, pattern):
             itw = tw.clone()
         ro = torch.zeros_like(a)
         assert O.ntt(a, prm.p, prm.perm, tw, ro) is None
-        rg = torch.zeros(a.shape, dtype=torch.int64, device="cuda")
+        rg = torch.zeros(a.shape, dtype=dtype, device="cuda")
         assert G.ntt(a.cuda(), prm.p.cuda(), prm.perm.cuda(), tw.cuda(), rg) is None
         assert torch.equal(rg.cpu(), ro), table
         io = torch.zeros_like(a)
-        ig = torch.zeros(a.shape, dtype=torch.int64, device="cuda")
+        ig = torch.zeros(a.shape, dtype=dtype, device="cuda")
         assert O.intt(ro, prm.p, prm.perm, itw, prm.m_inv, io) is None
         assert G.intt(rg, prm.p.cuda(), prm.perm.cuda(), itw.cuda(), prm.m_inv.cuda(), ig) is None
         assert torch.equal(ig.cpu(), io), table
